@@ -1,0 +1,217 @@
+/*
+ * lensed-amd: C ABI of the MI355X-native Lensed likelihood.
+ *
+ * Every entry point below replaces one of the reference's host interfaces;
+ * the comment on each cites the reference declaration (file:line) whose
+ * signature and meaning it keeps.  Plain C types only: no HIP or torch
+ * types cross this boundary.
+ */
+#ifndef LENSED_AMD_H
+#define LENSED_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+/* object types, src/input.h:4-9 */
+enum { OBJ_LENS = 'L', OBJ_SOURCE = 'S', OBJ_FOREGROUND = 'F' };
+
+/* parameter types, src/input.h:12-21 */
+enum {
+    PAR_PARAMETER = 0,
+    PAR_POSITION_X,
+    PAR_POSITION_Y,
+    PAR_RADIUS,
+    PAR_MAGNITUDE,
+    PAR_AXIS_RATIO,
+    PAR_POS_ANGLE
+};
+
+/* opaque prior, src/input.h:73 */
+typedef struct prior prior;
+
+/* parameter of an object, src/input.h:76-109 */
+typedef struct {
+    const char* name;
+    const char* id;
+    int type;
+    double lower;
+    double upper;
+    int bounded;
+    prior* pri;
+    int wrap;
+    int ipp;
+    int derived;
+    int defval;
+    const char* label;
+} param;
+
+/* object on the line of sight, src/input.h:112-129 (size in 4-byte words) */
+typedef struct {
+    int type;
+    size_t size;
+    const char* id;
+    const char* name;
+    size_t npars;
+    param* pars;
+} object;
+
+/* ------------------------------------------------------ program generator */
+
+/* src/kernel.h:5 -- sources of the per-object metadata program */
+void object_program(const char* name, size_t* nkernels, const char*** kernels);
+
+/* src/kernel.h:8 -- sources of the main program: prelude, objects, the
+ * generated compute()/set_params() and the device kernels (HIP C++ here) */
+void main_program(size_t nobjs, object objs[], size_t* nkernels, const char*** kernels);
+
+/* src/kernel.h:11-14 -- build options; the -D names are the reference's */
+const char* kernel_options(size_t width, size_t height, int psf, size_t psfw,
+                           size_t psfh, size_t nq, const char* flags[]);
+
+/* src/kernel.h:17 -- prefix + name, malloc'd */
+char* kernel_name(const char* prefix, const char* name);
+
+/* The model part of main_program() alone -- prelude, translated objects
+ * and the generated compute_0()/set_params_0() -- which also compiles with a
+ * host C++ compiler.  No counterpart in the reference; used to check the
+ * plugin translation and the generated ray-shoot on the CPU.  malloc'd,
+ * NULL on failure. */
+char* lensed_model_source(size_t nobjs, const object* objs);
+
+/* Build check with no counterpart in the reference (it compiles in
+ * clBuildProgram, src/lensed.c:755): compile a program from main_program()
+ * with hipRTC for `arch` (NULL = gfx950) without needing a device.  Returns
+ * the code-object size or -1; *log (malloc'd) receives the build log. */
+long lensed_program_compile(const char* source, const char* arch, char** log);
+
+/* ----------------------------------------------------------------- priors */
+
+/* src/prior.h:4-25 */
+prior* prior_read(const char* str);
+prior* prior_default(double value);
+void prior_free(prior* pri);
+void prior_print(const prior* pri, char* buf, size_t n);
+double prior_apply(const prior* pri, double u);
+double prior_lower(const prior* pri);
+double prior_upper(const prior* pri);
+int prior_pseudo(const prior* pri);
+
+/* -------------------------------------------------------- object metadata */
+
+/* Replaces the device round trip of add_object (src/input/objects.c:15-266):
+ * loads objects/<name>.cl, compiles its metadata program with hipRTC and
+ * reads type, data size and parameter table from the code object.  Fills
+ * obj (id, name, type, size, npars, pars with default pseudo-priors as
+ * src/input/objects.c:225-226).  Returns 0 on success. */
+int lensed_object_load(object* obj, const char* id, const char* name);
+void lensed_object_free(object* obj);
+
+/* ------------------------------------------------------------- quadrature */
+
+/* QUAD_RULES / quad_rule, src/quadrature.c:21-43.  qq, ww: 2*n floats each
+ * (node offset scaled by (sx, sy); value weight, error weight).  Returns the
+ * number of nodes, or -1 for an unknown rule. */
+int lensed_quad_size(const char* rule);
+int lensed_quad_rule(const char* rule, float* qq, float* ww, double sx, double sy);
+const char* lensed_quad_name(int index); /* NULL past the last rule */
+/* the unscaled rule on the unit pixel in double: pts 2*n, wht n, err n */
+int lensed_quad_table(const char* rule, double* pts, double* wht, double* err);
+
+/* -------------------------------------------------------------- context */
+
+/* the likelihood context, counterpart of struct lensed (src/lensed.h:3-72) */
+typedef struct lensed lensed;
+
+/* flags for lensed_config.flags */
+enum {
+    LENSED_PROFILE = 1, /* per-stage hipEvent timing, the --profile stages */
+    LENSED_EMULATE = 2, /* EPL angular factor from the trained emulator table */
+    LENSED_TABLE = 4,   /* EPL angular factor from the exact-series table */
+    LENSED_GRAPH = 8    /* replay batched launches from a captured hipGraph */
+};
+
+/* One observed system: data as prepared by src/lensed.c:406-511. */
+typedef struct {
+    size_t width, height;
+    const float* image;   /* [height][width], row-major, x fastest */
+    const float* weight;  /* [height][width]; 0 = masked */
+    const float* psf;     /* [psfh][psfw], normalised to unit sum; NULL = none */
+    size_t psfw, psfh;
+    double pcs[4];        /* pixel coordinates: origin (rx, ry), scale (sx, sy) */
+    const char* rule;     /* quadrature rule, NULL = "g3k7" */
+    size_t nobjs;
+    object* objs;         /* objects with priors set (lensed_object_load) */
+} lensed_system;
+
+typedef struct {
+    int device;           /* HIP device ordinal */
+    int flags;            /* LENSED_* */
+    size_t max_batch;     /* largest number of candidates per call */
+    const char* emulator; /* weights file for LENSED_EMULATE, or NULL */
+} lensed_options;
+
+/* Build the context for one or more systems (several systems are evaluated
+ * together in one launch).  NULL on failure, see lensed_last_error(). */
+lensed* lensed_create(const lensed_system* systems, size_t nsystems, const lensed_options* opts);
+void lensed_destroy(lensed* l);
+const char* lensed_last_error(void);
+
+/* parameter bookkeeping of system s, as src/lensed.c:106-271 builds it */
+size_t lensed_npars(const lensed* l, size_t s);
+size_t lensed_ndims(const lensed* l, size_t s);
+/* pmap[i] = object-order index of the i-th sampler parameter */
+int lensed_pmap(const lensed* l, size_t s, size_t* pmap);
+
+/* ---------------------------------------------------- sampler callbacks */
+
+/* src/nested.h:3 -- the MultiNest likelihood callback; the context is
+ * system 0 of a lensed*.  cube is in/out as in src/nested.c:43-61. */
+void loglike(double cube[], int* ndim, int* npar, double* lnew, void* lensed_);
+
+/* src/nested.h:4-6 -- progress callback; records the constraints and, when
+ * a dump prefix is set with lensed_set_output(), re-renders the ML point. */
+void dumper(int* nsamples, int* nlive, int* npar, double** physlive,
+            double** posterior, double** constraints, double* maxloglike,
+            double* logz, double* inslogz, double* logzerr, void* lensed_);
+
+/* ------------------------------------------------------ batch extension */
+
+/* ncand unit cubes (ndim each, sampler order) of system s -> physical
+ * parameters (npar each, sampler order, may be NULL) and log-likelihoods,
+ * evaluated in one batched launch.  Candidates whose transformed value
+ * falls outside hard bounds get logZero (-DBL_MAX) instead of hanging the
+ * way src/nested.c:55-57 does.  Returns 0 on success. */
+int lensed_loglike_batch(lensed* l, size_t s, const double* cubes, size_t ncand,
+                         double* phys_out, double* lnew_out);
+
+/* Physical parameters in object order ([ncand][npar]) -> log-likelihoods,
+ * for every system at once: phys[s], ncand[s], lnew[s]. */
+int lensed_loglike_systems(lensed* l, const double* const* phys, const size_t* ncand,
+                           double* const* lnew);
+
+/* One physical parameter vector (object order) of system s -> the planes
+ * the dumper writes (src/nested.c:176-237): raw value, quadrature error,
+ * PSF-convolved image and per-pixel chi^2.  Any output may be NULL. */
+int lensed_render(lensed* l, size_t s, const double* phys, float* value, float* error,
+                  float* image, float* chi2);
+
+/* stage timings in ms accumulated under LENSED_PROFILE, in the order
+ * +params, -params, set_params, render, convolve, loglike, +loglike_mem,
+ * -loglike_mem (src/lensed.c:1125-1132); returns the number of calls */
+long lensed_profile(const lensed* l, double* ms8);
+
+/* results recorded by dumper(): mean, sigma, ML, MAP per parameter in object
+ * order (each may be NULL) and the log-evidence summary */
+int lensed_results(const lensed* l, double* mean, double* sigma, double* ml, double* map,
+                   double* logev, double* logev_err, double* max_loglike);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LENSED_AMD_H */

@@ -1,0 +1,110 @@
+"""ctypes binding of include/lensed.h (the C ABI of liblensed_amd.so)."""
+import ctypes
+import os
+from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int, c_long, c_size_t,
+                    c_void_p)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblensed_amd.so")
+
+OBJ_LENS, OBJ_SOURCE, OBJ_FOREGROUND = ord("L"), ord("S"), ord("F")
+(PAR_PARAMETER, PAR_POSITION_X, PAR_POSITION_Y, PAR_RADIUS, PAR_MAGNITUDE, PAR_AXIS_RATIO,
+ PAR_POS_ANGLE) = range(7)
+LENSED_PROFILE, LENSED_EMULATE, LENSED_TABLE, LENSED_GRAPH = 1, 2, 4, 8
+
+
+class Param(Structure):
+    _fields_ = [("name", c_char_p), ("id", c_char_p), ("type", c_int), ("lower", c_double),
+                ("upper", c_double), ("bounded", c_int), ("pri", c_void_p), ("wrap", c_int),
+                ("ipp", c_int), ("derived", c_int), ("defval", c_int), ("label", c_char_p)]
+
+
+class Object(Structure):
+    _fields_ = [("type", c_int), ("size", c_size_t), ("id", c_char_p), ("name", c_char_p),
+                ("npars", c_size_t), ("pars", POINTER(Param))]
+
+
+class System(Structure):
+    _fields_ = [("width", c_size_t), ("height", c_size_t), ("image", POINTER(c_float)),
+                ("weight", POINTER(c_float)), ("psf", POINTER(c_float)), ("psfw", c_size_t),
+                ("psfh", c_size_t), ("pcs", c_double * 4), ("rule", c_char_p), ("nobjs", c_size_t),
+                ("objs", POINTER(Object))]
+
+
+class Options(Structure):
+    _fields_ = [("device", c_int), ("flags", c_int), ("max_batch", c_size_t), ("emulator", c_char_p)]
+
+
+# every entry point declared in include/lensed.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "object_program": (None, [c_char_p, POINTER(c_size_t), POINTER(POINTER(c_char_p))]),
+    "main_program": (None, [c_size_t, POINTER(Object), POINTER(c_size_t), POINTER(POINTER(c_char_p))]),
+    "kernel_options": (c_void_p, [c_size_t, c_size_t, c_int, c_size_t, c_size_t, c_size_t, POINTER(c_char_p)]),
+    "kernel_name": (c_void_p, [c_char_p, c_char_p]),
+    "lensed_program_compile": (c_long, [c_char_p, c_char_p, POINTER(c_void_p)]),
+    "lensed_model_source": (c_void_p, [c_size_t, POINTER(Object)]),
+    "prior_read": (c_void_p, [c_char_p]),
+    "prior_default": (c_void_p, [c_double]),
+    "prior_free": (None, [c_void_p]),
+    "prior_print": (None, [c_void_p, c_char_p, c_size_t]),
+    "prior_apply": (c_double, [c_void_p, c_double]),
+    "prior_lower": (c_double, [c_void_p]),
+    "prior_upper": (c_double, [c_void_p]),
+    "prior_pseudo": (c_int, [c_void_p]),
+    "lensed_object_load": (c_int, [POINTER(Object), c_char_p, c_char_p]),
+    "lensed_object_free": (None, [POINTER(Object)]),
+    "lensed_quad_size": (c_int, [c_char_p]),
+    "lensed_quad_rule": (c_int, [c_char_p, POINTER(c_float), POINTER(c_float), c_double, c_double]),
+    "lensed_quad_name": (c_char_p, [c_int]),
+    "lensed_quad_table": (c_int, [c_char_p, POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
+    "lensed_create": (c_void_p, [POINTER(System), c_size_t, POINTER(Options)]),
+    "lensed_destroy": (None, [c_void_p]),
+    "lensed_last_error": (c_char_p, []),
+    "lensed_npars": (c_size_t, [c_void_p, c_size_t]),
+    "lensed_ndims": (c_size_t, [c_void_p, c_size_t]),
+    "lensed_pmap": (c_int, [c_void_p, c_size_t, POINTER(c_size_t)]),
+    "loglike": (None, [POINTER(c_double), POINTER(c_int), POINTER(c_int), POINTER(c_double), c_void_p]),
+    "dumper": (None, [POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(POINTER(c_double)),
+                      POINTER(POINTER(c_double)), POINTER(POINTER(c_double)), POINTER(c_double),
+                      POINTER(c_double), POINTER(c_double), POINTER(c_double), c_void_p]),
+    "lensed_loglike_batch": (c_int, [c_void_p, c_size_t, POINTER(c_double), c_size_t, POINTER(c_double),
+                                     POINTER(c_double)]),
+    "lensed_loglike_systems": (c_int, [c_void_p, POINTER(POINTER(c_double)), POINTER(c_size_t),
+                                       POINTER(POINTER(c_double))]),
+    "lensed_render": (c_int, [c_void_p, c_size_t, POINTER(c_double), POINTER(c_float), POINTER(c_float),
+                              POINTER(c_float), POINTER(c_float)]),
+    "lensed_profile": (c_long, [c_void_p, POINTER(c_double)]),
+    "lensed_results": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                               POINTER(c_double), POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load liblensed_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("liblensed_amd.so is not built; run python -m lensed_amd.build")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.free = ctypes.CDLL(None).free
+        lib.free.argtypes = [c_void_p]
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def take_string(ptr):
+    """Copy and free a malloc'd C string returned by the library."""
+    lib = load()
+    s = ctypes.cast(ptr, c_char_p).value.decode()
+    lib.free(ptr)
+    return s
+
+
+def last_error():
+    return (load().lensed_last_error() or b"").decode()

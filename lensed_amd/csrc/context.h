@@ -1,0 +1,106 @@
+// The likelihood context (struct lensed) and its device runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/lensed.h"
+#include "../kernels/desc.h"
+#include "prior_impl.h"
+#include "program.h"
+
+namespace lsd {
+
+void set_error(const std::string& msg);
+
+// one parameter of a system, with its own copy of the prior
+struct Param {
+    std::string id;
+    int type = 0;
+    double lower = 0, upper = 0;
+    bool bounded = false;
+    prior pri_copy{prior::DELTA, 0, 0};  // copy of the caller's prior
+    bool has_prior = false;
+    bool ipp = false, derived = false, wrap = false;
+};
+
+struct System {
+    size_t width = 0, height = 0, npix = 0;
+    size_t psfw = 0, psfh = 0;
+    double pcs[4] = {1, 1, 1, 1};
+    std::string rule;
+    std::vector<float> quad;    // float4 nodes (dx, dy, w, e), scaled
+    int nq = 0;
+    std::vector<ModelObject> objs;
+    std::vector<Param> pars;    // object order
+    std::vector<size_t> pmap;   // sampler order -> object order
+    size_t ndims = 0;
+    int model = 0;
+    size_t dwords = 0;
+    // offsets of this system's constant data on the device
+    long long image_off = 0, psf_off = 0;
+    int quad_off = 0;
+    int ntile_r = 0, ntile_c = 0, tiles_x = 0;
+    // EPL emulation: word offset of the EPL object and indices of q, t
+    int epl_obj = -1, epl_q = -1, epl_t = -1;
+    // dumper results (object order)
+    std::vector<double> mean, sigma, ml, map;
+    double logev = 0, logev_err = 0, logev_ins = 0, max_loglike = 0;
+};
+
+// launch layout for a given number of candidates per system
+struct Layout {
+    std::vector<lensed_desc> desc;  // systems with ncand > 0
+    std::vector<int> sys_of_desc;
+    int n_set = 0, n_render = 0, n_conv = 0, n_final = 0;
+    size_t lds_conv = 0;
+    lensed_desc* d_desc = nullptr;
+    hipGraphExec_t graph = nullptr;
+};
+
+}  // namespace lsd
+
+struct lensed {
+    int device = 0;
+    int flags = 0;
+    size_t max_batch = 1;
+    std::vector<lsd::System> sys;
+    hipModule_t module = nullptr;
+    hipFunction_t f_set = nullptr, f_render = nullptr, f_conv = nullptr, f_final = nullptr;
+    hipStream_t stream = nullptr;
+    int grid_render = 0, grid_conv = 0;
+    // device buffers
+    float* d_image = nullptr;
+    float* d_weight = nullptr;
+    float* d_psf = nullptr;
+    float* d_quad = nullptr;
+    float* d_params = nullptr;
+    unsigned* d_blocks = nullptr;
+    float* d_model = nullptr;
+    float* d_err = nullptr;     // error plane, one candidate per system (dumper)
+    float* d_conv = nullptr;    // convolved image, one candidate (dumper)
+    float* d_chi2 = nullptr;    // per-pixel chi^2, one candidate (dumper)
+    double* d_partials = nullptr;
+    double* d_logl = nullptr;
+    // pinned host staging
+    float* h_params = nullptr;
+    double* h_logl = nullptr;
+    size_t cap_params = 0, cap_blocks = 0, cap_model = 0, cap_partials = 0, cap_out = 0;
+    std::map<std::vector<size_t>, lsd::Layout> layouts;
+    std::string source;          // generated program (for dumps)
+    // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem
+    hipEvent_t ev[9] = {};
+    double prof_ms[8] = {};
+    long prof_calls = 0;
+};
+
+namespace lsd {
+
+// evaluate ncand[s] parameter vectors (object order, float) per system
+int evaluate(::lensed* l, const std::vector<size_t>& ncand, const std::vector<const double*>& phys,
+             const std::vector<double*>& lnew);
+
+}  // namespace lsd

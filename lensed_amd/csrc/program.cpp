@@ -1,0 +1,326 @@
+// Plugin translation and generation of the device program's model code.
+//
+// The generated compute_<k>() follows the ray-shooting semantics of the code
+// Lensed generates (src/kernel.c:65-111, compute_kernel 235-399): objects are
+// visited in list order; deflections of consecutive lenses are summed at the
+// current ray position and applied when the list moves on to a source, a
+// non-finite total deflection sends the ray to 1e10; sources add their
+// brightness at the (possibly deflected) ray position; foregrounds always
+// use the image-plane position.  set_params_<k>() follows the parameter
+// setter (src/kernel.c:113-150, set_params_kernel 401-656), including
+// image-plane priors traced back through the lenses of earlier planes.
+#include "program.h"
+
+#include <dlfcn.h>
+#include <sys/stat.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+
+namespace lsd {
+
+std::string mangle(const std::string& name) {
+    std::string out;
+    for (char c : name) {
+        bool ok = (c >= 'a' && c <= 'z') || (c >= 'A' && c <= 'Z') || (c >= '0' && c <= '9') || c == '_';
+        out += ok ? c : '_';
+    }
+    if (out.empty() || (out[0] >= '0' && out[0] <= '9')) out = "_" + out;
+    return out;
+}
+
+static bool is_ident_start(char c) { return (c >= 'a' && c <= 'z') || (c >= 'A' && c <= 'Z') || c == '_'; }
+static bool is_ident_char(char c) { return is_ident_start(c) || (c >= '0' && c <= '9'); }
+
+static bool is_vector_type(const std::string& s) {
+    static const char* names[] = {"float2", "float3", "float4", "float8", "float16", "mat22",
+                                  "int2", "int3", "int4", "uint2", "uint3", "uint4", nullptr};
+    for (const char** n = names; *n; ++n)
+        if (s == *n) return true;
+    return false;
+}
+
+std::string translate_object(const std::string& src, const std::string& mangled) {
+    (void)mangled;
+    std::string out;
+    out.reserve(src.size() + 64);
+    const size_t n = src.size();
+    size_t i = 0;
+    auto skip_ws = [&](size_t j) {
+        while (j < n && (src[j] == ' ' || src[j] == '\t' || src[j] == '\n' || src[j] == '\r')) ++j;
+        return j;
+    };
+    while (i < n) {
+        char c = src[i];
+        if (c == '/' && i + 1 < n && src[i + 1] == '/') {  // line comment
+            size_t e = src.find('\n', i);
+            if (e == std::string::npos) e = n;
+            out.append(src, i, e - i);
+            i = e;
+        } else if (c == '/' && i + 1 < n && src[i + 1] == '*') {  // block comment
+            size_t e = src.find("*/", i + 2);
+            e = (e == std::string::npos) ? n : e + 2;
+            out.append(src, i, e - i);
+            i = e;
+        } else if (c == '"' || c == '\'') {  // string or character literal
+            size_t j = i + 1;
+            while (j < n && src[j] != c) j += (src[j] == '\\') ? 2 : 1;
+            j = j < n ? j + 1 : n;
+            out.append(src, i, j - i);
+            i = j;
+        } else if (is_ident_start(c) && (i == 0 || !is_ident_char(src[i - 1]))) {
+            size_t j = i;
+            while (j < n && is_ident_char(src[j])) ++j;
+            std::string id = src.substr(i, j - i);
+            if (id == "this")
+                out += "self";
+            else if (id == "local" || id == "global" || id == "private" || id == "__local" ||
+                     id == "__global" || id == "__private" || id == "__kernel")
+                ;  // address spaces: plain (generic) pointers in HIP C++
+            else if (id == "constant" || id == "__constant")
+                out += "const";
+            else
+                out += id;
+            i = j;
+        } else if (c >= '0' && c <= '9') {  // numeric literal, copied whole
+            size_t j = i;
+            while (j < n && (is_ident_char(src[j]) || src[j] == '.' ||
+                             ((src[j] == '+' || src[j] == '-') && (src[j - 1] == 'e' || src[j - 1] == 'E' ||
+                                                                   src[j - 1] == 'p' || src[j - 1] == 'P'))))
+                ++j;
+            out.append(src, i, j - i);
+            i = j;
+        } else if (c == '(') {
+            // OpenCL vector literal "(T)(a, b, ...)": in C++ that would be a
+            // comma expression cast to T, so rewrite it as the constructor
+            // call "T(a, b, ...)".  The argument list is left untouched.
+            size_t j = skip_ws(i + 1);
+            size_t k = j;
+            while (k < n && is_ident_char(src[k])) ++k;
+            std::string id = src.substr(j, k - j);
+            if (!id.empty() && is_vector_type(id)) {
+                size_t m = skip_ws(k);
+                if (m < n && src[m] == ')') {
+                    size_t p = skip_ws(m + 1);
+                    if (p < n && src[p] == '(') {
+                        out += id;
+                        i = p;
+                        continue;
+                    }
+                }
+            }
+            out += c;
+            ++i;
+        } else {
+            out += c;
+            ++i;
+        }
+    }
+    return out;
+}
+
+static const char* const kObjectMacros[] = {"type", "params", "data", "deflection", "brightness", "foreground", "set"};
+
+static std::string undef_macros() {
+    std::string s;
+    for (const char* m : kObjectMacros) s += std::string("#undef ") + m + "\n";
+    return s;
+}
+
+std::string wrap_object(const std::string& translated, const std::string& m, const std::string& name) {
+    std::ostringstream o;
+    o << "// ---------------------------------------------------------------- objects/" << name << ".cl\n"
+      << "#define type static const int type_" << m << "\n"
+      << "#define params static const struct param parlst_" << m << "[] =\n"
+      << "#define data struct data_" << m << "\n"
+      << "#define deflection deflection_" << m << "\n"
+      << "#define brightness brightness_" << m << "\n"
+      << "#define foreground foreground_" << m << "\n"
+      << "#define set set_" << m << "\n"
+      << "#line 1 \"objects/" << name << ".cl\"\n"
+      << translated << "\n"
+      << undef_macros();
+    return o.str();
+}
+
+std::string wrap_object_meta(const std::string& translated, const std::string& m, const std::string& name) {
+    std::ostringstream o;
+    o << "#define type extern \"C\" __device__ const int lensed_meta_type\n"
+      << "#define params extern \"C\" __device__ const struct param lensed_meta_params[] =\n"
+      << "#define data struct data_" << m << "\n"
+      << "#define deflection deflection_" << m << "\n"
+      << "#define brightness brightness_" << m << "\n"
+      << "#define foreground foreground_" << m << "\n"
+      << "#define set set_" << m << "\n"
+      << "#line 1 \"objects/" << name << ".cl\"\n"
+      << translated << "\n"
+      << undef_macros()
+      << "extern \"C\" __device__ const unsigned long long lensed_meta_size = sizeof(struct data_" << m << ");\n"
+      << "extern \"C\" __device__ const unsigned long long lensed_meta_align = alignof(struct data_" << m << ");\n"
+      << "extern \"C\" __device__ const unsigned long long lensed_meta_npar = "
+         "sizeof(lensed_meta_params)/sizeof(struct param);\n";
+    return o.str();
+}
+
+bool read_text(const std::string& path, std::string* out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::stringstream ss;
+    ss << f.rdbuf();
+    *out = ss.str();
+    return true;
+}
+
+static bool file_exists(const std::string& p) {
+    struct stat st;
+    return stat(p.c_str(), &st) == 0 && S_ISREG(st.st_mode);
+}
+
+// The package directory: the folder holding this shared library.
+static std::string package_dir() {
+    Dl_info info;
+    if (dladdr((void*)&package_dir, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        size_t s = p.rfind('/');
+        if (s != std::string::npos) return p.substr(0, s);
+    }
+    return ".";
+}
+
+std::string kernel_dir() {
+    const char* env = getenv("LENSED_KERNEL_PATH");
+    if (env && *env) return env;
+    return package_dir() + "/kernels";
+}
+
+bool load_object_source(const std::string& name, std::string* source, std::string* path) {
+    std::vector<std::string> dirs;
+    if (const char* env = getenv("LENSED_OBJECTS_PATH")) {
+        std::string s = env;
+        size_t b = 0;
+        while (b <= s.size()) {
+            size_t e = s.find(':', b);
+            if (e == std::string::npos) e = s.size();
+            if (e > b) dirs.push_back(s.substr(b, e - b));
+            b = e + 1;
+        }
+    }
+    dirs.push_back(package_dir() + "/objects");
+    for (const auto& d : dirs) {
+        std::string p = d + "/" + name + ".cl";
+        if (file_exists(p) && read_text(p, source)) {
+            if (path) *path = p;
+            return true;
+        }
+    }
+    return false;
+}
+
+size_t model_data_words(const std::vector<ModelObject>& objs) {
+    size_t d = 0;
+    for (const auto& o : objs) d += o.size;
+    return d;
+}
+
+static std::string data_ptr(const std::string& m, size_t off, const char* base) {
+    std::ostringstream o;
+    o << "(struct data_" << m << "*)(" << base << " + " << off << ")";
+    return o.str();
+}
+
+std::string compute_function(const std::vector<ModelObject>& objs, int k) {
+    std::ostringstream o;
+    o << "static inline float compute_" << k << "(uint* data, float2 x)\n{\n"
+      << "    float2 y = x;\n"
+      << "    float f = 0;\n";
+    bool plane_open = false;
+    int trigger = 0;
+    size_t d = 0;
+    auto close_plane = [&]() {
+        o << "        y -= dot(a, a) < HUGE_VALF ? a : float2(1E10f, 1E10f);\n    }\n";
+        plane_open = false;
+    };
+    for (const auto& ob : objs) {
+        std::string m = mangle(ob.name);
+        if (ob.type != trigger && ob.type != OBJ_FOREGROUND) {
+            if (trigger == OBJ_LENS && plane_open) close_plane();
+            trigger = ob.type;
+        }
+        if (ob.type == OBJ_LENS) {
+            if (!plane_open) {
+                o << "    {\n        float2 a = 0;\n";
+                plane_open = true;
+            }
+            o << "        a += deflection_" << m << "(" << data_ptr(m, d, "data") << ", y);\n";
+        } else if (ob.type == OBJ_SOURCE) {
+            o << "    f += brightness_" << m << "(" << data_ptr(m, d, "data") << ", y);\n";
+        } else if (ob.type == OBJ_FOREGROUND) {
+            o << "    f += foreground_" << m << "(" << data_ptr(m, d, "data") << ", x);\n";
+        }
+        d += ob.size;
+    }
+    if (plane_open) close_plane();
+    o << "    return f;\n}\n";
+    return o.str();
+}
+
+std::string set_params_function(const std::vector<ModelObject>& objs, int k) {
+    std::ostringstream o;
+    o << "static inline void set_params_" << k << "(uint* data, const float* params)\n{\n"
+      << "    float2 x = 0;\n"
+      << "    float2 a = 0;\n"
+      << "    (void)x; (void)a;\n";
+    int trigger = 0;
+    size_t plane = 0;
+    size_t d = 0, p = 0;
+    for (size_t i = 0; i < objs.size(); ++i) {
+        const auto& ob = objs[i];
+        std::string m = mangle(ob.name);
+        if (ob.type != trigger && ob.type != OBJ_FOREGROUND) {
+            if (trigger == OBJ_LENS && ob.type == OBJ_SOURCE) plane = i;
+            trigger = ob.type;
+        }
+        // image-plane priors: trace the observed position back through the
+        // lenses of all earlier planes
+        for (size_t j = 0; j < ob.npars; ++j) {
+            if (!ob.ipp[j] || ob.ptype[j] != PAR_POSITION_X) continue;
+            o << "    x = float2(params[" << p + j << "], params[" << p + j + 1 << "]);\n";
+            int trig2 = 0;
+            size_t d2 = 0;
+            for (size_t q = 0; q < plane; ++q) {
+                if (objs[q].type != trig2 && objs[q].type != OBJ_FOREGROUND) {
+                    if (trig2 == OBJ_LENS)
+                        o << "    x -= dot(a, a) < HUGE_VALF ? a : float2(1E10f, 1E10f);\n    a = 0;\n";
+                    trig2 = objs[q].type;
+                }
+                if (objs[q].type == OBJ_LENS) {
+                    std::string mq = mangle(objs[q].name);
+                    o << "    a += deflection_" << mq << "(" << data_ptr(mq, d2, "data") << ", x);\n";
+                }
+                d2 += objs[q].size;
+            }
+            if (trig2 == OBJ_LENS)
+                o << "    x -= dot(a, a) < HUGE_VALF ? a : float2(1E10f, 1E10f);\n    a = 0;\n";
+        }
+        o << "    set_" << m << "(" << data_ptr(m, d, "data");
+        for (size_t j = 0; j < ob.npars; ++j) {
+            if (ob.ipp[j] && ob.ptype[j] == PAR_POSITION_X)
+                o << ", x.x";
+            else if (ob.ipp[j] && ob.ptype[j] == PAR_POSITION_Y)
+                o << ", x.y";
+            else if (ob.ipp[j])
+                o << ", 0";
+            else
+                o << ", params[" << p + j << "]";
+        }
+        o << ");\n";
+        d += ob.size;
+        p += ob.npars;
+    }
+    o << "}\n";
+    return o.str();
+}
+
+}  // namespace lsd

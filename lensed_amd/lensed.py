@@ -1,0 +1,250 @@
+"""Host-side mirror of Lensed's likelihood setup, over the C ABI.
+
+A system is what one Lensed configuration describes (src/input/ini.c): an
+ordered list of objects (`[objects]`), priors per parameter (`[priors]`,
+docs/configuration.md:157-214), the observed image with its weights and an
+optional PSF (src/lensed.c:406-511) and the quadrature rule.  `Context`
+builds the device program for one or more systems and evaluates
+log-likelihoods through the same entry points a sampler uses: the
+MultiNest-style `loglike` callback (src/nested.c:17-131) one candidate at a
+time, or the batch extension many candidates per launch.
+
+All device work happens in liblensed_amd.so; nothing here computes a model.
+"""
+import ctypes
+from ctypes import POINTER, byref, c_double, c_float, c_int, c_size_t
+
+import numpy as np
+
+from . import _lib as L
+
+LOG_ZERO = -np.finfo(np.float64).max
+
+
+class LensedError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise LensedError(f"{what}: {L.last_error()}")
+
+
+class Prior:
+    """A prior in the reference syntax: a value, `unif a b` or `norm m s`."""
+
+    def __init__(self, text):
+        lib = L.load()
+        self.text = str(text)
+        self.ptr = lib.prior_read(self.text.encode())
+        if not self.ptr:
+            raise LensedError(f"invalid prior definition: {text}")
+
+    def apply(self, u):
+        return L.load().prior_apply(self.ptr, float(u))
+
+    @property
+    def lower(self):
+        return L.load().prior_lower(self.ptr)
+
+    @property
+    def upper(self):
+        return L.load().prior_upper(self.ptr)
+
+    @property
+    def pseudo(self):
+        return bool(L.load().prior_pseudo(self.ptr))
+
+    def __str__(self):
+        buf = ctypes.create_string_buffer(64)
+        L.load().prior_print(self.ptr, buf, 64)
+        return buf.value.decode()
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            try:
+                L.load().prior_free(self.ptr)
+            except Exception:
+                pass
+            self.ptr = None
+
+
+class LensedObject:
+    """One object of the line of sight, loaded from objects/<name>.cl."""
+
+    def __init__(self, oid, name):
+        lib = L.load()
+        self.obj = L.Object()
+        _check(lib.lensed_object_load(byref(self.obj), oid.encode(), name.encode()), f"object {oid}")
+        self.id, self.name = oid, name
+        self._priors = []
+
+    @property
+    def type(self):
+        return chr(self.obj.type)
+
+    @property
+    def params(self):
+        return [self.obj.pars[i].name.decode() for i in range(self.obj.npars)]
+
+    def set_prior(self, pname, text):
+        """`[wrap] [image] <prior>`, as the [priors] section takes it."""
+        words = str(text).split()
+        wrap = image = False
+        while words and words[0] in ("wrap", "image"):
+            wrap |= words[0] == "wrap"
+            image |= words[0] == "image"
+            words = words[1:]
+        names = self.params
+        if pname not in names:
+            raise LensedError(f"object {self.id} has no parameter {pname}")
+        par = self.obj.pars[names.index(pname)]
+        pri = Prior(" ".join(words))
+        lib = L.load()
+        if par.pri:
+            lib.prior_free(par.pri)
+        par.pri = lib.prior_read(pri.text.encode())  # owned by the object
+        par.wrap = int(wrap)
+        par.ipp = int(image)
+
+    def __del__(self):
+        if getattr(self, "obj", None) is not None:
+            try:
+                L.load().lensed_object_free(byref(self.obj))
+            except Exception:
+                pass
+            self.obj = None
+
+
+class SystemSpec:
+    """Objects, priors and data of one observed system."""
+
+    def __init__(self, objects, priors, image, weight, psf=None, pcs=(1.0, 1.0, 1.0, 1.0), rule="g3k7"):
+        self.objects = [o if isinstance(o, LensedObject) else LensedObject(*o) for o in objects]
+        byid = {o.id: o for o in self.objects}
+        for key, text in priors.items():
+            oid, pname = key.split(".", 1)
+            byid[oid].set_prior(pname, text)
+        self.image = np.ascontiguousarray(image, dtype=np.float32)
+        self.weight = np.ascontiguousarray(weight, dtype=np.float32)
+        if self.image.shape != self.weight.shape or self.image.ndim != 2:
+            raise LensedError("image and weight must be 2-D arrays of the same shape")
+        self.psf = None if psf is None else np.ascontiguousarray(psf, dtype=np.float32)
+        self.pcs = tuple(float(v) for v in pcs)
+        self.rule = rule
+
+    @property
+    def param_ids(self):
+        return [f"{o.id}.{p}" for o in self.objects for p in o.params]
+
+    def _struct(self):
+        s = L.System()
+        h, w = self.image.shape
+        s.width, s.height = w, h
+        s.image = self.image.ctypes.data_as(POINTER(c_float))
+        s.weight = self.weight.ctypes.data_as(POINTER(c_float))
+        if self.psf is not None:
+            s.psf = self.psf.ctypes.data_as(POINTER(c_float))
+            s.psfh, s.psfw = self.psf.shape
+        s.pcs = (c_double * 4)(*self.pcs)
+        s.rule = self.rule.encode()
+        self._objs = (L.Object * len(self.objects))(*[o.obj for o in self.objects])
+        s.nobjs = len(self.objects)
+        s.objs = self._objs
+        return s
+
+
+class Context:
+    """The likelihood context for one or more systems (struct lensed)."""
+
+    def __init__(self, systems, flags=0, max_batch=1, device=0):
+        if isinstance(systems, SystemSpec):
+            systems = [systems]
+        self.systems = list(systems)
+        lib = L.load()
+        structs = (L.System * len(self.systems))(*[s._struct() for s in self.systems])
+        opts = L.Options(device=device, flags=flags, max_batch=max_batch, emulator=None)
+        self.ptr = lib.lensed_create(structs, len(self.systems), byref(opts))
+        if not self.ptr:
+            raise LensedError(L.last_error())
+        self.max_batch = max_batch
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            L.load().lensed_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def npars(self, s=0):
+        return L.load().lensed_npars(self.ptr, s)
+
+    def ndims(self, s=0):
+        return L.load().lensed_ndims(self.ptr, s)
+
+    def pmap(self, s=0):
+        out = (c_size_t * self.npars(s))()
+        L.load().lensed_pmap(self.ptr, s, out)
+        return list(out)
+
+    def loglike(self, cube):
+        """One candidate through the MultiNest callback; returns (logL, physical)."""
+        npar = self.npars(0)
+        ndim = self.ndims(0)
+        buf = (c_double * npar)(*list(np.asarray(cube, dtype=np.float64)[:ndim]), *([0.0] * (npar - ndim)))
+        lnew = c_double()
+        L.load().loglike(buf, byref(c_int(ndim)), byref(c_int(npar)), byref(lnew), self.ptr)
+        return lnew.value, np.array(buf[:], dtype=np.float64)
+
+    def loglike_batch(self, cubes, s=0):
+        """ncand unit cubes -> (logL[ncand], physical[ncand, npar]) in one launch per max_batch."""
+        cubes = np.ascontiguousarray(cubes, dtype=np.float64)
+        n = cubes.shape[0]
+        phys = np.zeros((n, self.npars(s)))
+        lnew = np.zeros(n)
+        _check(L.load().lensed_loglike_batch(self.ptr, s, cubes.ctypes.data_as(POINTER(c_double)), n,
+                                             phys.ctypes.data_as(POINTER(c_double)),
+                                             lnew.ctypes.data_as(POINTER(c_double))), "loglike_batch")
+        return lnew, phys
+
+    def loglike_phys(self, phys_per_system):
+        """Physical parameters in object order, one [ncand, npar] array per system."""
+        if not isinstance(phys_per_system, (list, tuple)):
+            phys_per_system = [phys_per_system]
+        ns = len(self.systems)
+        arrs = [np.ascontiguousarray(np.atleast_2d(p), dtype=np.float64) for p in phys_per_system]
+        arrs += [np.zeros((0, self.npars(s))) for s in range(len(arrs), ns)]
+        outs = [np.zeros(a.shape[0]) for a in arrs]
+        pp = (POINTER(c_double) * ns)(*[a.ctypes.data_as(POINTER(c_double)) for a in arrs])
+        nc = (c_size_t * ns)(*[a.shape[0] for a in arrs])
+        oo = (POINTER(c_double) * ns)(*[o.ctypes.data_as(POINTER(c_double)) for o in outs])
+        _check(L.load().lensed_loglike_systems(self.ptr, pp, nc, oo), "loglike_systems")
+        return outs if len(phys_per_system) > 1 else outs[0]
+
+    def render(self, phys, s=0):
+        """Dumper planes for one physical parameter vector (object order)."""
+        spec = self.systems[s]
+        h, w = spec.image.shape
+        planes = {k: np.zeros((h, w), dtype=np.float32) for k in ("value", "error", "image", "chi2")}
+        p = np.ascontiguousarray(phys, dtype=np.float64)
+        fp = POINTER(c_float)
+        _check(L.load().lensed_render(self.ptr, s, p.ctypes.data_as(POINTER(c_double)),
+                                      planes["value"].ctypes.data_as(fp), planes["error"].ctypes.data_as(fp),
+                                      planes["image"].ctypes.data_as(fp), planes["chi2"].ctypes.data_as(fp)),
+               "render")
+        return planes
+
+    def profile(self):
+        """Accumulated stage times (ms) with the reference's stage names."""
+        ms = (c_double * 8)()
+        calls = L.load().lensed_profile(self.ptr, ms)
+        names = ["+params", "-params", "set_params", "render", "convolve", "loglike", "+loglike_mem",
+                 "-loglike_mem"]
+        return calls, dict(zip(names, list(ms)))

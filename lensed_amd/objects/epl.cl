@@ -1,0 +1,58 @@
+// Elliptical power-law lens (Tessore & Metcalf 2015).
+// In elliptical coordinates u (scaled by the Einstein radius) with polar angle
+// phi, the deflection is  n |u|^(1-t) * Omega(phi),  where the angular factor
+// Omega = sum_k A_k is the hypergeometric series with A_0 = e^{i phi} and
+// A_k = -f (2k - T)/(2k + T) e^{2 i phi} A_{k-1},  T = 2 - t, f = (1-q)/(1+q),
+// summed to tenth order.  cos(phi), sin(phi) are taken directly from u/|u|.
+type = LENS;
+
+params
+{
+    { "x",  POSITION_X              },
+    { "y",  POSITION_Y              },
+    { "r",  RADIUS                  },
+    { "t",  PARAMETER, { 0.f, 2.f } },
+    { "q",  AXIS_RATIO              },
+    { "pa", POS_ANGLE               }
+};
+
+data
+{
+    float2 c;       // centre
+    mat22 fwd;      // image frame -> elliptical frame, scaled by 1/(r sqrt(q))
+    mat22 bwd;      // elliptical frame -> image frame (rotation only)
+    float coef[10]; // series coefficients -f (2k - T)/(2k + T), k = 1..10
+    float hexp;     // (1 - t)/2, exponent applied to |u|^2
+    float norm;     // 2 r sqrt(q)/(1 + q)
+};
+
+static float2 deflection(local data* this, float2 x)
+{
+    float2 u = mv22(this->fwd, x - this->c);
+    float uu = dot(u, u);
+    float2 e = uu > 0 ? u*rsqrt(uu) : (float2)(1, 0);
+    float2 e2 = (float2)(e.x*e.x - e.y*e.y, 2*e.x*e.y);
+    float2 term = e;
+    float2 sum = e;
+    for(int k = 0; k < 10; ++k)
+    {
+        term = this->coef[k]*(float2)(e2.x*term.x - e2.y*term.y, e2.y*term.x + e2.x*term.y);
+        sum += term;
+    }
+    return mv22(this->bwd, (this->norm*powr(uu, this->hexp))*sum);
+}
+
+static void set(local data* this, float x, float y, float r, float t, float q, float pa)
+{
+    float cs;
+    float sn = sincos(pa*DEG2RAD, &cs);
+    float T = 2 - t;
+    float f = (1 - q)/(1 + q);
+    this->c = (float2)(x, y);
+    this->fwd = (1/r/sqrt(q))*(mat22)(q*cs, q*sn, -sn, cs);
+    this->bwd = (mat22)(cs, -sn, sn, cs);
+    for(int k = 1; k <= 10; ++k)
+        this->coef[k-1] = -f*(2*k - T)/(2*k + T);
+    this->hexp = 0.5f*(1 - t);
+    this->norm = 2*r*sqrt(q)/(1 + q);
+}

@@ -1,0 +1,42 @@
+// Sersic profile with the linear b(n) = 1.9992 n - 0.3271 (valid for
+// 0.5 < n < 8), the approximation Lensed used before its MacArthur-series
+// update.  Otherwise identical to the sersic object.
+type = SOURCE;
+
+params
+{
+    { "x",   POSITION_X },
+    { "y",   POSITION_Y },
+    { "r",   RADIUS     },
+    { "mag", MAGNITUDE  },
+    { "n",   PARAMETER, { 0.5f, 8.0f } },
+    { "q",   AXIS_RATIO },
+    { "pa",  POS_ANGLE  }
+};
+
+data
+{
+    float2 c;     // centre
+    mat22 tr;     // rotation and axis scaling
+    float lnorm;  // log of the central surface brightness
+    float lscale; // log(b) - log(r)/n
+    float hinv;   // 1/(2n)
+};
+
+static float brightness(local data* this, float2 x)
+{
+    float2 u = mv22(this->tr, x - this->c);
+    return exp(this->lnorm - exp(this->lscale + this->hinv*log(dot(u, u))));
+}
+
+static void set(local data* this, float x, float y, float r, float mag, float n, float q, float pa)
+{
+    float b = 1.9992f*n - 0.3271f;
+    float cs = cos(pa*DEG2RAD);
+    float sn = sin(pa*DEG2RAD);
+    this->c = (float2)(x, y);
+    this->tr = (mat22)(q*cs, q*sn, -sn, cs)/sqrt(q);
+    this->lnorm = -0.4f*mag*LOG_10 + 2*n*log(b) - LOG_PI - 2*log(r) - log(tgamma(2*n + 1));
+    this->lscale = log(b) - log(r)/n;
+    this->hinv = 0.5f/n;
+}

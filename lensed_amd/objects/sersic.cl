@@ -1,0 +1,49 @@
+// Sersic profile with effective radius r, index n and total magnitude mag.
+// I(R) = L/(2 pi n r^2 q) b^{2n}/Gamma(2n) exp(-b (R/r)^{1/n}) written as
+// exp(lnorm - exp(lscale + log(R^2)/(2n))).  b(n) from MacArthur, Courteau &
+// Holtzman (2003): the Ciotti & Bertin (1999) asymptotic series for n > 0.36,
+// their polynomial fit below.
+type = SOURCE;
+
+params
+{
+    { "x",   POSITION_X },
+    { "y",   POSITION_Y },
+    { "r",   RADIUS     },
+    { "mag", MAGNITUDE  },
+    { "n",   PARAMETER, POS_BOUND },
+    { "q",   AXIS_RATIO },
+    { "pa",  POS_ANGLE  }
+};
+
+data
+{
+    float2 c;     // centre
+    mat22 tr;     // rotation and axis scaling
+    float lnorm;  // log of the central surface brightness
+    float lscale; // log(b) - log(r)/n
+    float hinv;   // 1/(2n)
+};
+
+static float brightness(local data* this, float2 x)
+{
+    float2 u = mv22(this->tr, x - this->c);
+    return exp(this->lnorm - exp(this->lscale + this->hinv*log(dot(u, u))));
+}
+
+static void set(local data* this, float x, float y, float r, float mag, float n, float q, float pa)
+{
+    float b;
+    if(n > 0.36f)
+        b = 2.0f*n - 1.0f/3 + 4.0f/(405*n) + 46.0f/(25515*(n*n)) + 131.0f/(1148175*(n*n*n))
+          - 2194697.0f/(30690717750*(n*n*n*n));
+    else
+        b = 0.01945f - 0.8902f*n + 10.95f*(n*n) - 19.67f*(n*n*n) + 13.43f*(n*n*n*n);
+    float cs = cos(pa*DEG2RAD);
+    float sn = sin(pa*DEG2RAD);
+    this->c = (float2)(x, y);
+    this->tr = (mat22)(q*cs, q*sn, -sn, cs)/sqrt(q);
+    this->lnorm = -0.4f*mag*LOG_10 + 2*n*log(b) - LOG_PI - 2*log(r) - log(tgamma(2*n + 1));
+    this->lscale = log(b) - log(r)/n;
+    this->hinv = 0.5f/n;
+}
