@@ -1,0 +1,190 @@
+"""Benchmark: logL evaluations per second of the Lensed likelihood on MI355X.
+
+Workload (the configuration the metric is quoted on): 256 x 256 px, EPL +
+external shear lens, Gaussian source (sigma 0.1" = 2 px), 21 x 21 Gaussian
+PSF (FWHM 3 px), g3k7 quadrature (49 rays per pixel), mock data at the true
+parameters plus seeded noise (sigma 0.05).  A step is one batched launch of
+4096 candidate parameter vectors drawn uniformly from the priors: parameters
+in, one logL per candidate out (set_params -> render -> conv+chi^2 ->
+finalize, replayed from a captured hipGraph).
+
+Also reported: the plain GPU path (one candidate per call through the
+MultiNest `loglike` callback, the reference's launch structure), the render
+kernel's roofline position from hipEvent timings on its own stream, and the
+CPU restatement of the reference path timed on the host cores.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "logL evals/sec at 256² px, batched+emulated vs plain GPU, |ΔlogL|≤0.01"
+# algorithmic work per evaluation of this workload, counted from the
+# reference formulas (add/mul/div = 1, FMA = 2, transcendental = 1):
+# 165 flops per ray x 65536 px x 49 rays, PSF 441 MAC per pixel, chi^2 4 flops
+FLOP_RENDER = 165 * 65536 * 49
+FLOP_CONV = 2 * 441 * 65536
+FLOP_CHI2 = 4 * 65536
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X published FP32 (packed / matrix) peak
+PEAK_FP32_NONPACKED = 78.6
+PEAK_HBM = 8000.0  # GB/s
+
+
+def make_workload(K):
+    from lensed_amd import workloads as W
+    from lensed_amd.lensed import Context, LensedObject, SystemSpec
+
+    wl = W.c3()
+    objs = [LensedObject(oid, name) for oid, name in wl.objects]
+    pids = [f"{o.id}.{p}" for o in objs for p in o.params]
+
+    def render(objects, params, shape, psf):
+        pri = {k: repr(float(v)) for k, v in zip(pids, params)}
+        spec = SystemSpec(objects, pri, np.zeros(shape, np.float32), np.ones(shape, np.float32), psf=psf)
+        with Context(spec, max_batch=1) as c:
+            return c.render(params)["image"].astype(np.float64)
+
+    wl.make_data(render, pids, mag_keys=("source.mag",))
+    wl.ncand = K
+    return wl, pids
+
+
+def time_batched(wl, K, steps, warmup, flags):
+    from lensed_amd.lensed import Context
+    with Context(wl.spec(), max_batch=K, flags=flags) as ctx:
+        cubes = wl.candidates(ctx.ndims(), K)
+        for _ in range(warmup):
+            ctx.loglike_batch(cubes)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lnew, _ = ctx.loglike_batch(cubes)
+        dt = time.perf_counter() - t0
+    return dt, lnew
+
+
+def time_plain(wl, n):
+    from lensed_amd.lensed import Context
+    with Context(wl.spec(), max_batch=1) as ctx:
+        cubes = wl.candidates(ctx.ndims(), n)
+        for i in range(min(n, 20)):
+            ctx.loglike(cubes[i])
+        t0 = time.perf_counter()
+        out = [ctx.loglike(c)[0] for c in cubes]
+        dt = time.perf_counter() - t0
+    return n / dt, np.array(out)
+
+
+def kernel_times(wl, K, launches):
+    from lensed_amd._lib import LENSED_PROFILE
+    from lensed_amd.lensed import Context
+    with Context(wl.spec(), max_batch=K, flags=LENSED_PROFILE) as ctx:
+        cubes = wl.candidates(ctx.ndims(), K)
+        ctx.loglike_batch(cubes)
+        calls0, ms0 = ctx.profile()
+        for _ in range(launches):
+            ctx.loglike_batch(cubes)
+        calls, ms = ctx.profile()
+    n = calls - calls0
+    return {k: (ms[k] - ms0[k]) / n for k in ms}
+
+
+def cpu_baseline(wl, pids, seconds):
+    """The oracle's fast FP32 build (render + convolve + chi^2 per candidate,
+    OpenMP over pixels) on a bounded sample of the same candidates."""
+    from lensed_amd import quadrature as Q
+    from oracle import oracle as O
+    qq, ww = Q.rule("g3k7")
+    rng = np.random.default_rng(wl.cand_seed)
+    lo_hi = [tuple(map(float, wl.priors[k].split()[1:])) for k in pids]
+    done, t0 = 0, time.perf_counter()
+    image, weight = wl.image, wl.weight
+    while True:
+        u = rng.random(len(pids))
+        p = np.array([a + x * (b - a) for x, (a, b) in zip(u, lo_hi)], np.float32)
+        v, _ = O.render(wl.objects, p, wl.shape, qq, ww, dtype=np.float32, fast=True)
+        m = O.convolve(v, wl.psf, dtype=np.float32, fast=True)
+        O.chi2_f32terms(image, weight, m)
+        done += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds and done >= 3:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": done / dt, "unit": "logL evals/s", "cores": threads, "kind": "port",
+            "sample": f"{done} candidates of the same workload from the same priors, one at a time "
+                      f"({dt:.1f} s): oracle/liboracle_fast.so FP32, -O3 -ffast-math, OpenMP over pixels"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--plain", type=int, default=1000, help="candidates timed on the plain path")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: candidates are independent, so the path shards by splitting a "
+              "launch's candidates across GPUs (see DESIGN.md); multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from lensed_amd._lib import LENSED_GRAPH
+    K = args.batch
+    wl, pids = make_workload(K)
+
+    dt, lnew = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH)
+    value = args.steps * K / dt
+    plain_rate, lplain = time_plain(wl, args.plain)
+    # plain and batched use the same kernels: identical logL per candidate
+    same = bool(np.array_equal(lplain, lnew[: len(lplain)]))
+    kt = kernel_times(wl, K, 3)
+    render_s = kt["render"] / 1e3
+    achieved = FLOP_RENDER * K / render_s / 1e12
+    conv_s = kt["convolve"] / 1e3
+    cpu = cpu_baseline(wl, pids, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "logL evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "C3: 256x256 px, epl_plus_shear + gauss (sigma 2 px), 21x21 Gaussian PSF "
+                               "FWHM 3 px, g3k7 49 rays/px, sigma 0.05, candidates uniform from the priors",
+                   "candidates_per_launch": K, "path": "batched, exact EPL series, hipGraph replay",
+                   "seeds": {"noise": wl.seed, "candidates": wl.cand_seed}},
+        "plain_gpu": {"value": round(plain_rate, 2), "unit": "logL evals/s",
+                      "path": "one candidate per loglike() call, 4 kernels + 2 copies + sync",
+                      "identical_logL_to_batched": same},
+        "speedup_vs_plain": round(value / plain_rate, 3),
+        "roofline": {"bound": "valu", "kernel": "lensed_render", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
+                     "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED, 4),
+                     "traffic": None, "algorithmic_flop_per_launch": FLOP_RENDER * K,
+                     "avg_launch_ms": round(kt["render"], 4)},
+        "stages_ms_per_launch": {k: round(v, 4) for k, v in kt.items()},
+        "conv_chi2": {"achieved_tflops": round((FLOP_CONV + FLOP_CHI2) * K / conv_s / 1e12, 3),
+                      "avg_launch_ms": round(kt["convolve"], 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -70,7 +70,7 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
 #pragma unroll
-    for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = __builtin_amdgcn_readfirstlane(gb[i]);
+    for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = i < d.dwords ? __builtin_amdgcn_readfirstlane(gb[i]) : 0u;
     int k = tile * LENSED_RENDER_BS + (int)threadIdx.x;
     if (k >= d.npix) return;
     int py = k / d.width;

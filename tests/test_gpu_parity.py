@@ -1,0 +1,299 @@
+"""HIP path vs the CPU oracle, through the C ABI, on an MI355X.
+
+Bars (written per test): model images within 1e-4 relative of the FP64
+oracle wherever the model exceeds 1e-3 of its peak (and within 3e-5 of the
+peak everywhere for the reference renders); |delta logL| <= 1e-2 against
+the FP64 oracle near the truth, where logL differences decide acceptance;
+for arbitrary prior draws, whose |logL| reaches 1e5-1e7, the FP32 model
+rounding bounds the agreement at 1e-5 relative.  Plain and batched paths
+are compared bit for bit (same kernels, fixed reduction order).
+"""
+import numpy as np
+import pytest
+
+from conftest import fixture_objects
+from lensed_amd import quadrature as Q
+from lensed_amd import workloads as W
+from lensed_amd._lib import LENSED_GRAPH, LENSED_PROFILE
+from lensed_amd.lensed import LOG_ZERO, Context, SystemSpec
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PTS, WHT, ERR = Q.table("g3k7")
+WW64 = np.stack([WHT, ERR], 1)
+
+
+def oracle_model(objects, params, shape, psf=None, dtype=np.float64):
+    pcs = [1.0, 1.0, 1.0, 1.0]
+    if psf is not None:
+        pcs[0] += 0.5 * (psf.shape[1] % 2 == 0)
+        pcs[1] += 0.5 * (psf.shape[0] % 2 == 0)
+    if dtype == np.float64:
+        v, _ = O.render(objects, params, shape, PTS, WW64, pcs)
+    else:
+        qq, ww = Q.rule("g3k7")
+        v, _ = O.render(objects, params, shape, qq, ww, pcs, dtype=np.float32)
+    return O.convolve(v, psf, dtype) if psf is not None else v
+
+
+def oracle_render_cb(objects, params, shape, psf):
+    return oracle_model(objects, params, shape, psf)
+
+
+def rel_err(got, exact):
+    peak = np.abs(exact).max()
+    mask = np.abs(exact) > 1e-3 * peak
+    return np.max(np.abs(got - exact)[mask] / np.abs(exact[mask])), np.max(np.abs(got - exact)) / peak
+
+
+@pytest.fixture(scope="module")
+def fixture_ctx(golden):
+    meta, arrays = golden
+    names = sorted(meta)
+    specs = []
+    for t in names:
+        m = meta[t]
+        objs = fixture_objects(m)
+        pri = {k: repr(v) for k, v in m["params"].items()}
+        img = arrays[t + "/img"]
+        specs.append(SystemSpec(objs, pri, img, np.full_like(img, m["weight"])))
+    ctx = Context(specs, max_batch=2)
+    yield names, specs, ctx
+    ctx.close()
+
+
+def test_gpu_reproduces_reference_renders(golden, fixture_ctx):
+    meta, arrays = golden
+    names, specs, ctx = fixture_ctx
+    for s, t in enumerate(names):
+        objs = fixture_objects(meta[t])
+        p = O.param_vector(objs, meta[t]["params"])
+        planes = ctx.render(p, s)
+        img, err = arrays[t + "/img"], arrays[t + "/err"]
+        peak = np.abs(img).max()
+        mask = np.abs(img) > 1e-3 * peak
+        assert np.max(np.abs(planes["value"] - img)) <= 3e-5 * peak, t
+        exact = oracle_model(objs, p, img.shape)
+        rel, _ = rel_err(planes["value"], exact)
+        assert rel <= 1e-4, (t, rel)
+        assert np.max(np.abs(planes["error"][mask] / planes["value"][mask] - err[mask])) <= 2e-5, t
+        # no PSF: the image plane is the raw model, chi^2 per pixel w (m - d)^2
+        assert np.array_equal(planes["image"], planes["value"])
+        d = planes["value"] - img
+        assert np.allclose(planes["chi2"], (img * 0 + meta[t]["weight"]) * d * d, rtol=1e-6, atol=1e-12)
+
+
+def test_fixture_loglike_all_systems_one_launch(golden, fixture_ctx):
+    meta, arrays = golden
+    names, specs, ctx = fixture_ctx
+    phys = [O.param_vector(fixture_objects(meta[t]), meta[t]["params"])[None, :] for t in names]
+    out = ctx.loglike_phys(phys)
+    for s, t in enumerate(names):
+        objs = fixture_objects(meta[t])
+        img = arrays[t + "/img"]
+        model = oracle_model(objs, phys[s][0], img.shape)
+        lref = -0.5 * O.chi2_d(img, np.full_like(img, meta[t]["weight"]), model)
+        assert abs(out[s][0] - lref) <= 1e-2, (t, out[s][0], lref)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    wl = W.c1()
+    pids = [f"{oid}.{p}" for oid, n in wl.objects for p in O.PARAMS[n]]
+    wl.make_data(oracle_render_cb, pids, mag_keys=("source.mag",))
+    return wl, pids
+
+
+def test_c1_loglike_plain_batch_oracle(c1):
+    wl, pids = c1
+    with Context(wl.spec(), max_batch=64) as ctx:
+        assert ctx.npars() == 12 and ctx.ndims() == 12
+        cubes = wl.candidates(ctx.ndims(), 64)
+        cubes[0] = [(wl.truth[k] - lo) / (hi - lo) for k, (lo, hi) in
+                    ((k, map(float, wl.priors[k].split()[1:])) for k in pids)]
+        lb, pb = ctx.loglike_batch(cubes)
+        for i in range(8):
+            lp, pp = ctx.loglike(cubes[i])
+            assert lp == lb[i]
+            assert np.array_equal(pp, pb[i])
+        pmap = ctx.pmap()
+        for i in range(8):
+            phys = np.zeros(12)
+            phys[pmap] = pb[i]
+            model = oracle_model(wl.objects, phys, wl.shape)
+            lref = -0.5 * O.chi2_d(wl.image, wl.weight, model)
+            if i == 0:  # the truth: chi^2 ~ N
+                assert abs(lb[0] - lref) <= 1e-2, (lb[0], lref)
+                assert abs(-2 * lb[0] / wl.image.size - 1) < 0.05
+            assert abs(lb[i] - lref) <= 1e-2 + 1e-5 * abs(lref), (i, lb[i], lref)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    wl = W.c2()
+    pids = [f"{oid}.{p}" for oid, n in wl.objects for p in O.PARAMS[n]]
+    wl.make_data(oracle_render_cb, pids, mag_keys=("bulge.mag", "disk.mag"))
+    return wl, pids
+
+
+def test_c2_psf_convolution_and_loglike(c2):
+    wl, pids = c2
+    with Context(wl.spec(), max_batch=16) as ctx:
+        phys = np.array([wl.truth.get(k, 0.0) for k in pids])
+        planes = ctx.render(phys)
+        conv = O.convolve(planes["value"].astype(np.float64), wl.psf.astype(np.float64))
+        assert np.max(np.abs(planes["image"] - conv)) <= 2e-6 * np.abs(conv).max()
+        exact = oracle_model(wl.objects, phys, wl.shape, wl.psf)
+        rel, _ = rel_err(planes["image"], exact)
+        assert rel <= 1e-4, rel
+        lnew = ctx.loglike_phys(phys[None, :])[0]
+        lref = -0.5 * O.chi2_d(wl.image, wl.weight, exact)
+        assert abs(lnew - lref) <= 1e-2, (lnew, lref)
+
+
+@pytest.mark.parametrize("shape,psf_shape", [((37, 53), (5, 3)), ((64, 64), (4, 4)), ((33, 70), (6, 5)),
+                                             ((100, 40), (31, 31)), ((20, 20), (1, 1))])
+def test_ragged_images_and_even_psfs(shape, psf_shape):
+    rng = np.random.default_rng(sum(shape) + sum(psf_shape))
+    objs = [("lens", "sie"), ("src", "gauss")]
+    h, w = shape
+    p = np.array([w / 2 + 0.3, h / 2 - 0.2, min(h, w) / 5, 0.7, 33.0, w / 2 + 1.0, h / 2, 2.0, 0.0, 0.8, 10.0])
+    psf = rng.random(psf_shape).astype(np.float32)
+    psf /= psf.sum()
+    image = rng.normal(0, 0.1, shape).astype(np.float32)
+    weight = rng.uniform(0, 100, shape).astype(np.float32)
+    weight[0, :3] = 0  # masked pixels
+    pri = {f"{oid}.{n}": repr(float(v)) for (oid, n), v in
+           zip([(o, n) for o, name in objs for n in O.PARAMS[name]], p)}
+    with Context(SystemSpec(objs, pri, image, weight, psf=psf), max_batch=4) as ctx:
+        planes = ctx.render(p)
+        pcs = [1.0 + 0.5 * (psf_shape[1] % 2 == 0), 1.0 + 0.5 * (psf_shape[0] % 2 == 0), 1.0, 1.0]
+        raw, _ = O.render(objs, p, shape, PTS, WW64, pcs)
+        rel, _ = rel_err(planes["value"], raw)
+        assert rel <= 1e-4
+        conv = O.convolve(planes["value"].astype(np.float64), psf.astype(np.float64))
+        assert np.max(np.abs(planes["image"] - conv)) <= 2e-6 * np.abs(conv).max()
+        lnew, _ = ctx.loglike([])
+        lref = -0.5 * O.chi2_f32terms(image, weight, planes["image"])
+        assert abs(lnew - lref) <= 1e-6 * abs(lref) + 1e-6
+
+
+def test_multi_system_launch_equals_single():
+    rng = np.random.default_rng(5)
+    specs, phys = [], []
+    for n, lens, psf in [(64, "sie", None), (96, "epl", W.gaussian_psf(15, 3.0)), (128, "sie", W.gaussian_psf(31, 3.0))]:
+        objs = [("lens", lens), ("src", "sersic")]
+        names = [f"{o}.{p}" for o, nm in objs for p in O.PARAMS[nm]]
+        base = {"lens.x": n / 2, "lens.y": n / 2, "lens.r": n / 6, "lens.q": 0.7, "lens.pa": 20.0, "lens.t": 1.1,
+                "src.x": n / 2 + 1, "src.y": n / 2, "src.r": 3.0, "src.mag": 0.0, "src.n": 2.0, "src.q": 0.8,
+                "src.pa": 5.0}
+        pri = {k: f"unif {base[k] * 0.95 - 0.5} {base[k] * 1.05 + 0.5}" for k in names}
+        pri["lens.q"] = "unif 0.5 0.9"
+        img = rng.normal(0, 0.05, (n, n)).astype(np.float32)
+        specs.append(SystemSpec(objs, pri, img, np.full_like(img, 400.0), psf=psf))
+        phys.append(np.array([[base[k] * (1 + 0.01 * rng.normal()) for k in names] for _ in range(5)]))
+    with Context(specs, max_batch=8) as ctx:
+        together = ctx.loglike_phys(phys)
+    for s, spec in enumerate(specs):
+        with Context(spec, max_batch=8) as one:
+            alone = one.loglike_phys(phys[s])
+        assert np.array_equal(together[s], alone)
+
+
+def test_graph_replay_matches_eager(c1):
+    wl, pids = c1
+    cubes = wl.candidates(12, 200)
+    with Context(wl.spec(), max_batch=256) as eager:
+        le, _ = eager.loglike_batch(cubes)
+    with Context(wl.spec(), max_batch=256, flags=LENSED_GRAPH) as g:
+        for rep in range(3):
+            lg, _ = g.loglike_batch(cubes)
+            assert np.array_equal(lg, le), rep
+            lp, _ = g.loglike(cubes[rep])  # eager work between replays
+            assert lp == le[rep]
+        lg2, _ = g.loglike_batch(cubes[::-1].copy())
+        assert np.array_equal(lg2, le[::-1])
+
+
+def test_chunking_beyond_max_batch(c1):
+    wl, _ = c1
+    cubes = wl.candidates(12, 37)
+    with Context(wl.spec(), max_batch=8) as ctx:
+        l1, p1 = ctx.loglike_batch(cubes)
+        l2 = np.array([ctx.loglike(c)[0] for c in cubes])
+    assert np.array_equal(l1, l2)
+
+
+def test_out_of_bounds_prior_gives_logzero(c1):
+    wl, _ = c1
+    pri = dict(wl.priors)
+    pri["lens.q"] = "unif 0.5 1.5"  # partly outside the axis-ratio bounds (0, 1)
+    with Context(wl.spec(pri), max_batch=64) as ctx:
+        cubes = wl.candidates(12, 64)
+        lnew, phys = ctx.loglike_batch(cubes)
+        q = phys[:, ctx.pmap().index(3)]
+        assert np.all(lnew[q > 1] == LOG_ZERO)
+        assert np.all(np.isfinite(lnew[q <= 1]))
+        assert (q > 1).any() and (q <= 1).any()
+
+
+def test_nonfinite_parameters_do_not_poison_later_calls(c1):
+    wl, pids = c1
+    with Context(wl.spec(), max_batch=4) as ctx:
+        good = np.array([wl.truth[k] for k in pids])
+        l0 = ctx.loglike_phys(good[None])[0]
+        bad = good.copy()
+        bad[2] = np.nan
+        ctx.loglike_phys(np.stack([bad, good]))
+        assert ctx.loglike_phys(good[None])[0] == l0
+
+
+def test_profile_stages(c1):
+    wl, _ = c1
+    with Context(wl.spec(), max_batch=16, flags=LENSED_PROFILE) as ctx:
+        ctx.loglike_batch(wl.candidates(12, 16))
+        ctx.loglike_batch(wl.candidates(12, 16))
+        calls, ms = ctx.profile()
+    assert calls == 2
+    assert list(ms) == ["+params", "-params", "set_params", "render", "convolve", "loglike", "+loglike_mem",
+                        "-loglike_mem"]
+    assert ms["render"] > 0
+
+
+@pytest.fixture(scope="module")
+def c3():
+    wl = W.c3()
+    pids = [f"{oid}.{p}" for oid, n in wl.objects for p in O.PARAMS[n]]
+    wl.make_data(oracle_render_cb, pids, mag_keys=("source.mag",))
+    return wl, pids
+
+
+def test_c3_full_size_batch_properties(c3):
+    """256 x 256, 21x21 PSF, 4096 candidates per launch: size-independent
+    properties of the whole batch plus oracle checks of a sample."""
+    wl, pids = c3
+    with Context(wl.spec(), max_batch=4096, flags=LENSED_GRAPH) as ctx:
+        cubes = wl.candidates(ctx.ndims(), 4096)
+        lnew, phys = ctx.loglike_batch(cubes)
+        assert np.all(np.isfinite(lnew)) and np.all(lnew <= 0)
+        perm = np.random.default_rng(0).permutation(4096)
+        lperm, _ = ctx.loglike_batch(cubes[perm])
+        assert np.array_equal(lperm, lnew[perm])  # no cross-candidate interference
+        truth = np.array([wl.truth[k] for k in pids])
+        lt = ctx.loglike_phys(truth[None])[0]
+        n = wl.image.size
+        assert abs(-2 * lt / n - 1) < 0.03  # chi^2/dof ~ 1 at the truth
+        assert lt > lnew.max()
+        pmap = ctx.pmap()
+        for i in [0, 1, 2, 4095]:
+            p = np.zeros(len(pids))
+            p[pmap] = phys[i]
+            model = oracle_model(wl.objects, p, wl.shape, wl.psf)
+            lref = -0.5 * O.chi2_d(wl.image, wl.weight, model)
+            assert abs(lnew[i] - lref) <= 1e-2 + 1e-5 * abs(lref), (i, lnew[i], lref)
+        model = oracle_model(wl.objects, truth, wl.shape, wl.psf)
+        assert abs(lt - (-0.5 * O.chi2_d(wl.image, wl.weight, model))) <= 1e-2
+        planes = ctx.render(truth)
+        rel, _ = rel_err(planes["image"], model)
+        assert rel <= 1e-4, rel
