@@ -71,15 +71,28 @@ def time_batched(wl, K, steps, warmup, flags):
 
 
 def time_plain(wl, n):
+    """The sampler's calling pattern: one C loglike() callback per candidate
+    (ctypes arguments prebuilt, so the loop adds ~1 us per call)."""
+    import ctypes
+    from ctypes import byref, c_double, c_int
+    from lensed_amd import _lib as L
     from lensed_amd.lensed import Context
+    lib = L.load()
     with Context(wl.spec(), max_batch=1) as ctx:
-        cubes = wl.candidates(ctx.ndims(), n)
-        for i in range(min(n, 20)):
-            ctx.loglike(cubes[i])
+        ndim, npar = ctx.ndims(), ctx.npars()
+        cubes = wl.candidates(ndim, n)
+        bufs = [(c_double * npar)(*cubes[i], *([0.0] * (npar - ndim))) for i in range(n)]
+        warm = [(c_double * npar)(*cubes[i], *([0.0] * (npar - ndim))) for i in range(min(n, 20))]
+        nd, npr, ptr = byref(c_int(ndim)), byref(c_int(npar)), ctypes.c_void_p(ctx.ptr)
+        lnew = [c_double() for _ in range(n)]
+        for b in warm:
+            lib.loglike(b, nd, npr, byref(c_double()), ptr)
+        f = lib.loglike
         t0 = time.perf_counter()
-        out = [ctx.loglike(c)[0] for c in cubes]
+        for i in range(n):
+            f(bufs[i], nd, npr, lnew[i], ptr)
         dt = time.perf_counter() - t0
-    return n / dt, np.array(out)
+    return n / dt, np.array([v.value for v in lnew])
 
 
 def kernel_times(wl, K, launches):

@@ -43,7 +43,10 @@ struct System {
     // offsets of this system's constant data on the device
     long long image_off = 0, psf_off = 0;
     int quad_off = 0;
-    int ntile_r = 0, ntile_c = 0, tiles_x = 0;
+    int ntile_r = 0;
+    int sep_rank = 0;             // separable PSF rank, 0 = direct convolution
+    long long sep_off = 0;        // offset of the PSF factors in the psf buffer
+    long long psft_off = 0;       // offset of the transposed PSF taps
     // EPL emulation: word offset of the EPL object and indices of q, t
     int epl_obj = -1, epl_q = -1, epl_t = -1;
     // dumper results (object order)
@@ -57,6 +60,7 @@ struct Layout {
     std::vector<int> sys_of_desc;
     int n_set = 0, n_render = 0, n_conv = 0, n_final = 0;
     size_t lds_conv = 0;
+    bool conv_small = false;        // short tiles: launch too small to fill the GPU
     lensed_desc* d_desc = nullptr;
     hipGraphExec_t graph = nullptr;
 };
@@ -69,9 +73,10 @@ struct lensed {
     size_t max_batch = 1;
     std::vector<lsd::System> sys;
     hipModule_t module = nullptr;
-    hipFunction_t f_set = nullptr, f_render = nullptr, f_conv = nullptr, f_final = nullptr;
+    hipFunction_t f_set = nullptr, f_render = nullptr, f_conv = nullptr, f_conv_small = nullptr, f_final = nullptr;
+    int cu_count = 1;
     hipStream_t stream = nullptr;
-    int grid_render = 0, grid_conv = 0;
+    int grid_render = 0, grid_conv = 0, grid_conv_small = 0;
     // device buffers
     float* d_image = nullptr;
     float* d_weight = nullptr;

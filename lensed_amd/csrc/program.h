@@ -63,6 +63,13 @@ std::string model_source(const std::vector<std::vector<ModelObject>>& models, st
 
 // Assemble the whole device program (prelude, objects, model code, kernels)
 // for a list of models; empty string and *err on failure.
-std::string assemble_program(const std::vector<std::vector<ModelObject>>& models, std::string* err);
+// psfs: the (width, height, separable) PSF shapes that get compile-time
+// specialised convolution code; any other shape runs the generic path.
+struct PsfShape {
+    int w, h, sep;
+    bool operator==(const PsfShape& o) const { return w == o.w && h == o.h && sep == o.sep; }
+};
+std::string assemble_program(const std::vector<std::vector<ModelObject>>& models, std::string* err,
+                             const std::vector<PsfShape>& psfs = {});
 
 }  // namespace lsd

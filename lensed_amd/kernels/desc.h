@@ -6,8 +6,10 @@
 #define LENSED_RENDER_BS 256  // threads (= pixels) per render work item
 #define LENSED_CONV_TX 64     // conv tile width  (threads along x)
 #define LENSED_CONV_TY 4      // threads along y
-#define LENSED_CONV_RY 8      // output rows per thread
-#define LENSED_CONV_TH (LENSED_CONV_TY * LENSED_CONV_RY)  // conv tile height
+#define LENSED_CONV_RY 8      // output rows per thread, large launches
+#define LENSED_CONV_RY_SMALL 2  // output rows per thread, launches too small to fill the GPU
+// chi^2 partial sums are formed over fixed 8-row x 64-column blocks, so the
+// short tile (4 x 2 rows) is exactly one block and the tall one (4 x 8) four
 #define LENSED_CONV_BS (LENSED_CONV_TX * LENSED_CONV_TY)
 #define LENSED_PSF_MAX 63     // largest PSF side the conv tile is sized for
 
@@ -26,5 +28,12 @@ struct lensed_desc {
     long long table_off;             // float2 into the EPL angular tables ([ncand][...])
     int item_r, item_c, item_s, item_f;  // first work item in render, conv, set, final
     int item_t, epl_obj, epl_q, epl_t;   // table items; EPL object word offset, q and t param index
-    int quad_off, nq, pad0, pad1;        // node offset into quad[] (float4 units), node count
+    int quad_off, nq, conv_ry, sep_rank;  // nodes (float4 offset, count); conv rows/thread; PSF rank (0 = direct)
+    long long sep_off;                   // floats into psf[]: per rank, column [psfh] then row [psfw] factor
+    int npart, pad2, pad3, pad4;         // chi^2 partials per candidate: 8-row blocks x 64-column segments
+    long long psft_off;                  // floats into psf[]: the taps transposed, [psfw][psfh]
 };
+
+// LDS row stride of the conv tile: 64 + PW - 1 columns plus room for the
+// vector reads of the row pass, a multiple of 4 floats
+#define LENSED_TILE_STRIDE(PW) (((LENSED_CONV_TX + (PW) - 1 + 3) + 3) / 4 * 4)

@@ -15,6 +15,9 @@
 #ifndef LENSED_DWORDS
 #error "generated model code missing"
 #endif
+#ifndef LENSED_FOR_PSFS
+#define LENSED_FOR_PSFS(X)
+#endif
 
 typedef unsigned int u32;
 
@@ -62,7 +65,7 @@ extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
 // One thread per pixel; the 2-D quadrature loop runs in the thread with the
 // node table in scalar registers and the candidate's object block held in
 // SGPRs (wave-uniform), so the per-ray work is pure VALU.
-template <int K>
+template <int K, bool ERR>
 __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int tile,
                                           const u32* __restrict__ blocks, const float4* __restrict__ quad,
                                           float* __restrict__ model, float* __restrict__ errp)
@@ -84,11 +87,11 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
         const float4 node = q[n];
         float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y));
         f += node.z * v;
-        e += node.w * v;
+        if (ERR) e += node.w * v;
     }
     long long o = d.model_off + (long long)cand * d.npix + k;
     model[o] = f;
-    if (errp) errp[o] = e;
+    if (ERR) errp[o] = e;
 }
 
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
@@ -103,7 +106,11 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
         int cand = local / d.ntile_r;
         int tile = local - cand * d.ntile_r;
         switch (d.model) {
-#define LENSED_RENDER_CASE(K) case K: lensed_render_item<K>(d, cand, tile, blocks, quad, model, errp); break;
+#define LENSED_RENDER_CASE(K)                                                                    \
+    case K:                                                                                      \
+        if (errp) lensed_render_item<K, true>(d, cand, tile, blocks, quad, model, errp);         \
+        else lensed_render_item<K, false>(d, cand, tile, blocks, quad, model, errp);             \
+        break;
             LENSED_FOR_MODELS(LENSED_RENDER_CASE)
 #undef LENSED_RENDER_CASE
             default: break;
@@ -112,15 +119,265 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
 }
 
 // --------------------------------------------------------------- conv+chi2
-// Direct convolution with the flipped PSF and clamp-to-edge borders, as
+// Convolution with the flipped PSF and clamp-to-edge borders, as
 // kernel/lensed.cl:56-103: out(y, x) = sum_{j,i} psf[j][i] *
-// in(clamp(y + PH-1-PH/2 - j), clamp(x + PW-1-PW/2 - i)).  A 64x32 output
-// tile plus halo is staged in LDS; each thread produces 8 rows of one column.
-// The chi^2 term w (m - d)^2 is formed per pixel in FP32 (kernel/lensed.cl:50-51),
-// accumulated per thread with Kahan compensation and combined in FP64.
+// in(clamp(y + PH-1-PH/2 - j), clamp(x + PW-1-PW/2 - i)).  An output tile of
+// 64 x (4*RY) pixels plus halo is staged in LDS; each thread produces RY rows
+// of one column.  PSFs that are numerically a sum of few outer products
+// (sep_rank > 0, factors from the host SVD) go through row and column passes;
+// the others are summed directly.  The chi^2 term w (m - d)^2 is formed per
+// pixel in FP32 (kernel/lensed.cl:50-51) and summed with compensation over
+// fixed blocks of 8 rows x 64 columns: TwoSum down each column, then a TwoSum
+// tree across the 64 lanes, so each block yields the FP32 pair (sum, error)
+// of its terms' total; blocks are combined in FP64 in a fixed order.  The
+// block is the same whatever the tile height, so every launch shape gives
+// the same bits for the same candidate.
 __device__ inline double lensed_wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+__device__ inline void lensed_two_sum_add(float& s, float& c, float x) {
+#pragma clang fp reassociate(off) contract(off)
+    float t = s + x;  // TwoSum(s, x): t + e == s + x exactly
+    float bb = t - s;
+    float e = (s - (t - bb)) + (x - bb);
+    s = t;
+    c += e;
+}
+
+__device__ inline double lensed_wave_csum(float s, float c) {
+#pragma clang fp reassociate(off) contract(off)
+    for (int o = 32; o > 0; o >>= 1) {
+        float s2 = __shfl_xor(s, o, 64);
+        float c2 = __shfl_xor(c, o, 64);
+        float t = s + s2;  // TwoSum(s, s2): t + e == s + s2 exactly
+        float bb = t - s;
+        float e = (s - (t - bb)) + (s2 - bb);
+        s = t;
+        c = (c + c2) + e;
+    }
+    return (double)s + (double)c;
+}
+
+// Stage the model tile plus halo (clamped to the image) in LDS; row stride
+// LENSED_TILE_STRIDE(PW) keeps rows 16-byte aligned for vector reads.
+template <int RY>
+__device__ inline void lensed_conv_load_tile(const lensed_desc& d, const float* __restrict__ m, float* tile,
+                                             int tx0, int ty0, int lx, int ly, int PW, int PH)
+{
+    const int W = d.width, H = d.height;
+    const int cw = LENSED_CONV_TX + PW - 1, ch = LENSED_CONV_TY * RY + PH - 1;
+    const int stride = LENSED_TILE_STRIDE(PW);
+    const int ox = tx0 - PW / 2, oy = ty0 - PH / 2;
+    for (int r = ly; r < ch; r += LENSED_CONV_TY) {
+        const float* src = m + min(max(oy + r, 0), H - 1) * W;
+        for (int c = lx; c < cw; c += LENSED_CONV_TX) tile[r * stride + c] = src[min(max(ox + c, 0), W - 1)];
+    }
+}
+
+// PSF of any size, sizes from the descriptor
+template <int RY>
+__device__ inline void lensed_conv_generic(const lensed_desc& d, const float* __restrict__ psf, float* tile,
+                                           int lx, int ly, float (&acc)[RY])
+{
+    constexpr int TX = LENSED_CONV_TX, TH = LENSED_CONV_TY * RY, BS = LENSED_CONV_BS;
+    const int PW = d.psfw, PH = d.psfh;
+    const int stride = LENSED_TILE_STRIDE(PW), ch = TH + PH - 1;
+    if (d.sep_rank > 0) {
+        float* tmp = tile + stride * ch;  // row-pass results, ch x TX
+        for (int k = 0; k < d.sep_rank; ++k) {
+            const float* cf = psf + d.sep_off + (long long)k * (PH + PW);
+            const float* rf = cf + PH;
+            for (int idx = threadIdx.x; idx < ch * TX; idx += BS) {
+                int r = idx / TX, x = idx - r * TX;
+                const float* src = tile + r * stride + x + PW - 1;
+                float sum = 0.0f;
+                for (int i = 0; i < PW; ++i) sum += rf[i] * src[-i];
+                tmp[idx] = sum;
+            }
+            __syncthreads();
+            for (int j = 0; j < PH; ++j) {
+                const float w = cf[j];
+                const float* col = tmp + (ly * RY + PH - 1 - j) * TX + lx;
+#pragma unroll
+                for (int r = 0; r < RY; ++r) acc[r] += w * col[r * TX];
+            }
+            __syncthreads();
+        }
+    } else {
+        const float* p = psf + d.psf_off;
+        for (int j = 0; j < PH; ++j) {
+            const float* row = tile + (ly * RY + PH - 1 - j) * stride + lx + PW - 1;
+            for (int i = 0; i < PW; ++i) {
+                const float w = p[j * PW + i];
+#pragma unroll
+                for (int r = 0; r < RY; ++r) acc[r] += w * row[r * stride - i];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// PSF size fixed at compile time: taps unrolled into scalar registers and
+// every LDS value reused from VGPRs.  Separable: the row pass computes four
+// adjacent outputs per thread from one vector read of 4 + PW - 1 values;
+// the column pass RY outputs from RY + PH - 1 values.  Direct: per PSF
+// column, RY + PH - 1 values feed RY x PH multiply-adds (taps stored
+// transposed so a column's taps are contiguous).
+template <int RY, int PW, int PH, bool SEP>
+__device__ inline void lensed_conv_fixed(const lensed_desc& d, const float* __restrict__ psf, float* tile,
+                                         int lx, int ly, float (&acc)[RY])
+{
+    constexpr int TX = LENSED_CONV_TX, TH = LENSED_CONV_TY * RY, BS = LENSED_CONV_BS;
+    constexpr int CH = TH + PH - 1, STRIDE = LENSED_TILE_STRIDE(PW), NC = RY + PH - 1;
+    if (SEP) {
+        constexpr int NX = 4, NV = (NX + PW - 1 + 3) / 4;
+        float* tmp = tile + STRIDE * CH;
+        for (int k = 0; k < d.sep_rank; ++k) {
+            const float* cf = psf + d.sep_off + (long long)k * (PH + PW);
+            const float* rf = cf + PH;
+            for (int r = threadIdx.x / (TX / NX); r < CH; r += BS / (TX / NX)) {
+                const int x0 = (threadIdx.x % (TX / NX)) * NX;
+                const float4* src = (const float4*)(tile + r * STRIDE + x0);
+                float v[4 * NV];
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    float4 f = src[q];
+                    v[4 * q] = f.x; v[4 * q + 1] = f.y; v[4 * q + 2] = f.z; v[4 * q + 3] = f.w;
+                }
+                float o[NX];
+#pragma unroll
+                for (int x = 0; x < NX; ++x) o[x] = 0.0f;
+#pragma unroll
+                for (int i = 0; i < PW; ++i)
+#pragma unroll
+                    for (int x = 0; x < NX; ++x) o[x] += rf[i] * v[x + PW - 1 - i];
+                *(float4*)(tmp + r * TX + x0) = make_float4(o[0], o[1], o[2], o[3]);
+            }
+            __syncthreads();
+            float col[NC];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) col[q] = tmp[(ly * RY + q) * TX + lx];
+#pragma unroll
+            for (int j = 0; j < PH; ++j)
+#pragma unroll
+                for (int r = 0; r < RY; ++r) acc[r] += cf[j] * col[r + PH - 1 - j];
+            __syncthreads();
+        }
+    } else {
+        const float* pt = psf + d.psft_off;  // [PW][PH]
+#pragma unroll 1
+        for (int i = 0; i < PW; ++i) {
+            float col[NC];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) col[q] = tile[(ly * RY + q) * STRIDE + lx + PW - 1 - i];
+#pragma unroll
+            for (int j = 0; j < PH; ++j)
+#pragma unroll
+                for (int r = 0; r < RY; ++r) acc[r] += pt[i * PH + j] * col[r + PH - 1 - j];
+        }
+        __syncthreads();
+    }
+}
+
+template <int RY>
+__device__ inline void lensed_conv_item(const lensed_desc& d, int cand, int t,
+                                        const float* __restrict__ model, const float* __restrict__ image,
+                                        const float* __restrict__ weight, const float* __restrict__ psf,
+                                        double* __restrict__ partials, float* __restrict__ conv_out,
+                                        float* __restrict__ chi2_out, float* tile, double* red, float* rows8)
+{
+#pragma clang fp reassociate(off)
+    constexpr int TX = LENSED_CONV_TX, TH = LENSED_CONV_TY * RY;
+    const int lx = threadIdx.x % TX;
+    const int ly = threadIdx.x / TX;
+    const int W = d.width, H = d.height;
+    const int tx0 = (t % d.tiles_x) * TX;
+    const int ty0 = (t / d.tiles_x) * TH;
+    const float* m = model + d.model_off + (long long)cand * d.npix;
+    const float* img = image + d.image_off;
+    const float* wgt = weight + d.image_off;
+    const int gx = tx0 + lx;
+    float acc[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) acc[r] = 0.0f;
+    if (d.psfw > 0) {
+        lensed_conv_load_tile<RY>(d, m, tile, tx0, ty0, lx, ly, d.psfw, d.psfh);
+        __syncthreads();
+        bool done = false;
+#define LENSED_PSF_CASE(PW_, PH_, SEP_)                                                  \
+        if (!done && d.psfw == PW_ && d.psfh == PH_ && (d.sep_rank > 0) == SEP_) {       \
+            lensed_conv_fixed<RY, PW_, PH_, SEP_>(d, psf, tile, lx, ly, acc);            \
+            done = true;                                                                 \
+        }
+        LENSED_FOR_PSFS(LENSED_PSF_CASE)
+#undef LENSED_PSF_CASE
+        if (!done) lensed_conv_generic<RY>(d, psf, tile, lx, ly, acc);
+    } else {
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            int gy = ty0 + ly * RY + r;
+            acc[r] = (gx < W && gy < H) ? m[gy * W + gx] : 0.0f;
+        }
+    }
+    float chi[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int gy = ty0 + ly * RY + r;  // wave-uniform
+        chi[r] = 0.0f;
+        if (gx < W && gy < H) {
+            int kk = gy * W + gx;
+            float dm = acc[r] - img[kk];
+            chi[r] = wgt[kk] * dm * dm;
+            if (conv_out) conv_out[d.model_off + (long long)cand * d.npix + kk] = acc[r];
+            if (chi2_out) chi2_out[d.model_off + (long long)cand * d.npix + kk] = chi[r];
+        }
+    }
+    // canonical 8-row blocks: a thread of the tall tile owns one block
+    // column; the short tile's 4 threads pass their rows through LDS
+    float s = 0.0f, c = 0.0f;
+    int block_row;
+    if (RY == 8) {
+#pragma unroll
+        for (int r = 0; r < RY; ++r) lensed_two_sum_add(s, c, chi[r]);
+        block_row = (ty0 + ly * RY) / 8;
+    } else {
+#pragma unroll
+        for (int r = 0; r < RY; ++r) rows8[(ly * RY + r) * TX + lx] = chi[r];
+        __syncthreads();
+        if (ly == 0)
+            for (int r = 0; r < 8; ++r) lensed_two_sum_add(s, c, rows8[r * TX + lx]);
+        block_row = ty0 / 8;
+    }
+    if (RY == 8 || ly == 0) {
+        double blk = lensed_wave_csum(s, c);
+        if (lx == 0 && block_row * 8 < H)
+            partials[d.partial_off + (long long)cand * d.npart + (long long)block_row * d.tiles_x + t % d.tiles_x] = blk;
+    }
+    if (RY != 8) __syncthreads();  // rows8 is reused by the next item
+    (void)red;
+}
+
+template <int RY>
+__device__ inline void lensed_conv_chi2_body(const lensed_desc* __restrict__ sys, int nsys, int nitems,
+                                             const float* __restrict__ model, const float* __restrict__ image,
+                                             const float* __restrict__ weight, const float* __restrict__ psf,
+                                             double* __restrict__ partials, float* __restrict__ conv_out,
+                                             float* __restrict__ chi2_out)
+{
+    extern __shared__ float4 tile4[];
+    float* tile = (float*)tile4;
+    __shared__ double red[LENSED_CONV_BS / 64];
+    __shared__ float rows8[8 * LENSED_CONV_TX];
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        int s = lensed_find_sys<1>(sys, nsys, item);
+        const lensed_desc& d = sys[s];
+        int local = item - d.item_c;
+        int cand = local / d.ntile_c;
+        int t = local - cand * d.ntile_c;
+        lensed_conv_item<RY>(d, cand, t, model, image, weight, psf, partials, conv_out, chi2_out, tile, red, rows8);
+    }
 }
 
 extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) void lensed_conv_chi2(
@@ -129,86 +386,22 @@ extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) void lensed_conv_chi2(
     const float* __restrict__ weight, const float* __restrict__ psf,
     double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out)
 {
-#pragma clang fp reassociate(off)
-    extern __shared__ float tile[];
-    __shared__ double red[LENSED_CONV_BS / 64];
-    const int lx = threadIdx.x % LENSED_CONV_TX;
-    const int ly = threadIdx.x / LENSED_CONV_TX;
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-        int s = lensed_find_sys<1>(sys, nsys, item);
-        const lensed_desc& d = sys[s];
-        int local = item - d.item_c;
-        int cand = local / d.ntile_c;
-        int t = local - cand * d.ntile_c;
-        const int W = d.width, H = d.height;
-        const int tx0 = (t % d.tiles_x) * LENSED_CONV_TX;
-        const int ty0 = (t / d.tiles_x) * LENSED_CONV_TH;
-        const float* m = model + d.model_off + (long long)cand * d.npix;
-        const float* img = image + d.image_off;
-        const float* wgt = weight + d.image_off;
-        const int gx = tx0 + lx;
-        float acc[LENSED_CONV_RY];
-        if (d.psfw > 0) {
-            const int PW = d.psfw, PH = d.psfh;
-            const int cw = LENSED_CONV_TX + PW - 1, ch = LENSED_CONV_TH + PH - 1;
-            const int ox = tx0 - PW / 2, oy = ty0 - PH / 2;
-            for (int idx = threadIdx.x; idx < cw * ch; idx += LENSED_CONV_BS) {
-                int r = idx / cw, c = idx - r * cw;
-                int yy = min(max(oy + r, 0), H - 1);
-                int xx = min(max(ox + c, 0), W - 1);
-                tile[idx] = m[yy * W + xx];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < LENSED_CONV_RY; ++r) acc[r] = 0.0f;
-            const float* p = psf + d.psf_off;
-            for (int j = 0; j < PH; ++j) {
-                const float* row = tile + (ly * LENSED_CONV_RY + PH - 1 - j) * cw + lx + PW - 1;
-                for (int i = 0; i < PW; ++i) {
-                    const float w = p[j * PW + i];
-#pragma unroll
-                    for (int r = 0; r < LENSED_CONV_RY; ++r) acc[r] += w * row[r * cw - i];
-                }
-            }
-            __syncthreads();
-        } else {
-#pragma unroll
-            for (int r = 0; r < LENSED_CONV_RY; ++r) {
-                int gy = ty0 + ly * LENSED_CONV_RY + r;
-                acc[r] = (gx < W && gy < H) ? m[gy * W + gx] : 0.0f;
-            }
-        }
-        float sum = 0.0f, comp = 0.0f;
-#pragma unroll
-        for (int r = 0; r < LENSED_CONV_RY; ++r) {
-            int gy = ty0 + ly * LENSED_CONV_RY + r;
-            if (gx < W && gy < H) {
-                int kk = gy * W + gx;
-                float dm = acc[r] - img[kk];
-                float chi = wgt[kk] * dm * dm;
-                float yk = chi - comp;
-                float tk = sum + yk;
-                comp = (tk - sum) - yk;
-                sum = tk;
-                if (conv_out) conv_out[d.model_off + (long long)cand * d.npix + kk] = acc[r];
-                if (chi2_out) chi2_out[d.model_off + (long long)cand * d.npix + kk] = chi;
-            }
-        }
-        double v = lensed_wave_sum((double)sum - (double)comp);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double tot = 0.0;
-#pragma unroll
-            for (int w = 0; w < LENSED_CONV_BS / 64; ++w) tot += red[w];
-            partials[d.partial_off + (long long)cand * d.ntile_c + t] = tot;
-        }
-        __syncthreads();
-    }
+    lensed_conv_chi2_body<LENSED_CONV_RY>(sys, nsys, nitems, model, image, weight, psf, partials, conv_out,
+                                          chi2_out);
+}
+
+extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) void lensed_conv_chi2_small(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems,
+    const float* __restrict__ model, const float* __restrict__ image,
+    const float* __restrict__ weight, const float* __restrict__ psf,
+    double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out)
+{
+    lensed_conv_chi2_body<LENSED_CONV_RY_SMALL>(sys, nsys, nitems, model, image, weight, psf, partials,
+                                                conv_out, chi2_out);
 }
 
 // ---------------------------------------------------------------- finalize
-// One wave per candidate: FP64 sum of the tile partials in a fixed order,
+// One wave per candidate: FP64 sum of the row-segment partials in a fixed order,
 // logL = -chi^2/2 (src/nested.c:115).
 extern "C" __global__ __launch_bounds__(256) void lensed_finalize(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
@@ -220,9 +413,9 @@ extern "C" __global__ __launch_bounds__(256) void lensed_finalize(
     int s = lensed_find_sys<3>(sys, nsys, item);
     const lensed_desc& d = sys[s];
     int cand = item - d.item_f;
-    const double* p = partials + d.partial_off + (long long)cand * d.ntile_c;
+    const double* p = partials + d.partial_off + (long long)cand * d.npart;
     double acc = 0.0;
-    for (int t = lane; t < d.ntile_c; t += 64) acc += p[t];
+    for (int t = lane; t < d.npart; t += 64) acc += p[t];
     acc = lensed_wave_sum(acc);
     if (lane == 0) logl[d.out_off + cand] = -0.5 * acc;
 }

@@ -21,7 +21,7 @@ data
     mat22 fwd;      // image frame -> elliptical frame, scaled by 1/(r sqrt(q))
     mat22 bwd;      // elliptical frame -> image frame (rotation only)
     mat22 shear;    // shear matrix
-    float coef[10]; // series coefficients -f (2k - T)/(2k + T), k = 1..10
+    float coef[10]; // C_k = prod_{j<=k} -f (2j - T)/(2j + T), k = 1..10
     float hexp;     // (1 - t)/2, exponent applied to |u|^2
     float norm;     // 2 r sqrt(q)/(1 + q)
 };
@@ -31,15 +31,14 @@ static float2 deflection(local data* this, float2 x)
     float2 d = x - this->c;
     float2 u = mv22(this->fwd, d);
     float uu = dot(u, u);
-    float2 e = uu > 0 ? u*rsqrt(uu) : (float2)(1, 0);
-    float2 e2 = (float2)(e.x*e.x - e.y*e.y, 2*e.x*e.y);
-    float2 term = e;
-    float2 sum = e;
-    for(int k = 0; k < 10; ++k)
-    {
-        term = this->coef[k]*(float2)(e2.x*term.x - e2.y*term.y, e2.y*term.x + e2.x*term.y);
-        sum += term;
-    }
+    float ir = rsqrt(uu);
+    float2 e = (float2)(uu > 0 ? u.x*ir : 1.0f, uu > 0 ? u.y*ir : 0.0f);
+    float2 z = (float2)(e.x*e.x - e.y*e.y, 2*e.x*e.y);
+    float2 p = (float2)(this->coef[9], 0);
+    for(int k = 8; k >= 0; --k)
+        p = (float2)(this->coef[k] + z.x*p.x - z.y*p.y, z.y*p.x + z.x*p.y);
+    p = (float2)(1 + z.x*p.x - z.y*p.y, z.y*p.x + z.x*p.y);
+    float2 sum = (float2)(e.x*p.x - e.y*p.y, e.y*p.x + e.x*p.y);
     return mv22(this->bwd, (this->norm*powr(uu, this->hexp))*sum) + mv22(this->shear, d);
 }
 
@@ -53,8 +52,12 @@ static void set(local data* this, float x, float y, float r, float t, float q, f
     this->fwd = (1/r/sqrt(q))*(mat22)(q*cs, q*sn, -sn, cs);
     this->bwd = (mat22)(cs, -sn, sn, cs);
     this->shear = (mat22)(g1, g2, g2, -g1);
+    float c = 1;
     for(int k = 1; k <= 10; ++k)
-        this->coef[k-1] = -f*(2*k - T)/(2*k + T);
+    {
+        c *= -f*(2*k - T)/(2*k + T);
+        this->coef[k-1] = c;
+    }
     this->hexp = 0.5f*(1 - t);
     this->norm = 2*r*sqrt(q)/(1 + q);
 }

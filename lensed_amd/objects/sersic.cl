@@ -1,6 +1,6 @@
 // Sersic profile with effective radius r, index n and total magnitude mag.
 // I(R) = L/(2 pi n r^2 q) b^{2n}/Gamma(2n) exp(-b (R/r)^{1/n}) written as
-// exp(lnorm - exp(lscale + log(R^2)/(2n))).  b(n) from MacArthur, Courteau &
+// exp(lnorm - exp(lscale + log(R^2)/(2n))), evaluated in base 2.  b(n) from MacArthur, Courteau &
 // Holtzman (2003): the Ciotti & Bertin (1999) asymptotic series for n > 0.36,
 // their polynomial fit below.
 type = SOURCE;
@@ -20,15 +20,15 @@ data
 {
     float2 c;     // centre
     mat22 tr;     // rotation and axis scaling
-    float lnorm;  // log of the central surface brightness
-    float lscale; // log(b) - log(r)/n
+    float lnorm;  // log2 of the central surface brightness
+    float lscale; // (log(b) - log(r)/n) log2(e)
     float hinv;   // 1/(2n)
 };
 
 static float brightness(local data* this, float2 x)
 {
     float2 u = mv22(this->tr, x - this->c);
-    return exp(this->lnorm - exp(this->lscale + this->hinv*log(dot(u, u))));
+    return exp2(this->lnorm - 1.44269504088896340736f*exp2(this->lscale + this->hinv*log2(dot(u, u))));
 }
 
 static void set(local data* this, float x, float y, float r, float mag, float n, float q, float pa)
@@ -43,7 +43,7 @@ static void set(local data* this, float x, float y, float r, float mag, float n,
     float sn = sin(pa*DEG2RAD);
     this->c = (float2)(x, y);
     this->tr = (mat22)(q*cs, q*sn, -sn, cs)/sqrt(q);
-    this->lnorm = -0.4f*mag*LOG_10 + 2*n*log(b) - LOG_PI - 2*log(r) - log(tgamma(2*n + 1));
-    this->lscale = log(b) - log(r)/n;
+    this->lnorm = (-0.4f*mag*LOG_10 + 2*n*log(b) - LOG_PI - 2*log(r) - log(tgamma(2*n + 1)))*1.44269504088896340736f;
+    this->lscale = (log(b) - log(r)/n)*1.44269504088896340736f;
     this->hinv = 0.5f/n;
 }

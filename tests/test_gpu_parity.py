@@ -173,7 +173,10 @@ def test_ragged_images_and_even_psfs(shape, psf_shape):
         rel, _ = rel_err(planes["value"], raw)
         assert rel <= 1e-4
         conv = O.convolve(planes["value"].astype(np.float64), psf.astype(np.float64))
-        assert np.max(np.abs(planes["image"] - conv)) <= 2e-6 * np.abs(conv).max()
+        # FP32 sum of PW*PH taps in another order than the FP64 reference:
+        # rounding grows like sqrt(taps) * eps; a wrong tap or index is O(1e-2)
+        tol = 8 * np.sqrt(psf.size) * np.finfo(np.float32).eps
+        assert np.max(np.abs(planes["image"] - conv)) <= tol * np.abs(conv).max()
         lnew, _ = ctx.loglike([])
         lref = -0.5 * O.chi2_f32terms(image, weight, planes["image"])
         assert abs(lnew - lref) <= 1e-6 * abs(lref) + 1e-6
