@@ -47,8 +47,9 @@ struct System {
     int sep_rank = 0;             // separable PSF rank, 0 = direct convolution
     long long sep_off = 0;        // offset of the PSF factors in the psf buffer
     long long psft_off = 0;       // offset of the transposed PSF taps
-    // EPL emulation: word offset of the EPL object and indices of q, t
+    // EPL angular table: index of the EPL object and of its q, t parameters
     int epl_obj = -1, epl_q = -1, epl_t = -1;
+    bool table = false;
     // dumper results (object order)
     std::vector<double> mean, sigma, ml, map;
     double logev = 0, logev_err = 0, logev_ins = 0, max_loglike = 0;
@@ -58,8 +59,8 @@ struct System {
 struct Layout {
     std::vector<lensed_desc> desc;  // systems with ncand > 0
     std::vector<int> sys_of_desc;
-    int n_set = 0, n_render = 0, n_conv = 0, n_final = 0;
-    size_t lds_conv = 0;
+    int n_set = 0, n_render = 0, n_conv = 0, n_final = 0, n_table = 0;
+    size_t lds_conv = 0, lds_render = 0;
     bool conv_small = false;        // short tiles: launch too small to fill the GPU
     lensed_desc* d_desc = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -73,7 +74,8 @@ struct lensed {
     size_t max_batch = 1;
     std::vector<lsd::System> sys;
     hipModule_t module = nullptr;
-    hipFunction_t f_set = nullptr, f_render = nullptr, f_conv = nullptr, f_conv_small = nullptr, f_final = nullptr;
+    hipFunction_t f_set = nullptr, f_render = nullptr, f_conv = nullptr, f_conv_small = nullptr, f_final = nullptr,
+                  f_tables = nullptr;
     int cu_count = 1;
     hipStream_t stream = nullptr;
     int grid_render = 0, grid_conv = 0, grid_conv_small = 0;
@@ -90,10 +92,11 @@ struct lensed {
     float* d_chi2 = nullptr;    // per-pixel chi^2, one candidate (dumper)
     double* d_partials = nullptr;
     double* d_logl = nullptr;
+    void* d_tables = nullptr;   // EPL angular tables, float4 [candidates][LENSED_EPL_TABLE_F4]
     // pinned host staging
     float* h_params = nullptr;
     double* h_logl = nullptr;
-    size_t cap_params = 0, cap_blocks = 0, cap_model = 0, cap_partials = 0, cap_out = 0;
+    size_t cap_params = 0, cap_blocks = 0, cap_model = 0, cap_partials = 0, cap_out = 0, cap_tables = 0;
     std::map<std::vector<size_t>, lsd::Layout> layouts;
     std::string source;          // generated program (for dumps)
     // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem

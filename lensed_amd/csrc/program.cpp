@@ -232,7 +232,8 @@ static std::string data_ptr(const std::string& m, size_t off, const char* base) 
 
 std::string compute_function(const std::vector<ModelObject>& objs, int k) {
     std::ostringstream o;
-    o << "static inline float compute_" << k << "(uint* data, float2 x)\n{\n"
+    o << "static inline float compute_" << k << "(uint* data, float2 x, const float4* tab)\n{\n"
+      << "    (void)tab;\n"
       << "    float2 y = x;\n"
       << "    float f = 0;\n";
     bool plane_open = false;
@@ -253,7 +254,11 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
                 o << "    {\n        float2 a = 0;\n";
                 plane_open = true;
             }
-            o << "        a += deflection_" << m << "(" << data_ptr(m, d, "data") << ", y);\n";
+            if (ob.table)
+                o << "        a += lensed_epl_table_deflection" << (ob.name == "epl_plus_shear" ? "_shear" : "")
+                  << "(" << data_ptr(m, d, "data") << ", y, tab);\n";
+            else
+                o << "        a += deflection_" << m << "(" << data_ptr(m, d, "data") << ", y);\n";
         } else if (ob.type == OBJ_SOURCE) {
             o << "    f += brightness_" << m << "(" << data_ptr(m, d, "data") << ", y);\n";
         } else if (ob.type == OBJ_FOREGROUND) {

@@ -47,6 +47,7 @@ struct ModelObject {
     size_t npars;
     std::vector<int> ptype; // parameter types
     std::vector<int> ipp;   // image-plane prior flags
+    bool table = false;     // EPL deflection with the tabulated angular factor
 };
 
 // Generated code for one model (ordered object list): compute_<k>() and

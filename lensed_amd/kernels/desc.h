@@ -12,6 +12,8 @@
 // short tile (4 x 2 rows) is exactly one block and the tall one (4 x 8) four
 #define LENSED_CONV_BS (LENSED_CONV_TX * LENSED_CONV_TY)
 #define LENSED_PSF_MAX 63     // largest PSF side the conv tile is sized for
+#define LENSED_EPL_NT 128     // EPL angular table: intervals per octant
+#define LENSED_EPL_TABLE_F4 (2 * LENSED_EPL_NT * 2)  // float4 per candidate (2 octants x 2 components)
 
 struct lensed_desc {
     int width, height, npix, model;  // image size, pixels, model index
@@ -30,7 +32,7 @@ struct lensed_desc {
     int item_t, epl_obj, epl_q, epl_t;   // table items; EPL object word offset, q and t param index
     int quad_off, nq, conv_ry, sep_rank;  // nodes (float4 offset, count); conv rows/thread; PSF rank (0 = direct)
     long long sep_off;                   // floats into psf[]: per rank, column [psfh] then row [psfw] factor
-    int npart, pad2, pad3, pad4;         // chi^2 partials per candidate: 8-row blocks x 64-column segments
+    int npart, table_mode, pad3, pad4;   // chi^2 partials per candidate (8-row x 64-column blocks); EPL table in use
     long long psft_off;                  // floats into psf[]: the taps transposed, [psfw][psfh]
 };
 

@@ -27,7 +27,8 @@ __device__ inline int lensed_find_sys(const lensed_desc* __restrict__ sys, int n
     int lo = 0, hi = nsys - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        int first = WHICH == 0 ? sys[mid].item_r : WHICH == 1 ? sys[mid].item_c : WHICH == 2 ? sys[mid].item_s : sys[mid].item_f;
+        int first = WHICH == 0 ? sys[mid].item_r : WHICH == 1 ? sys[mid].item_c : WHICH == 2 ? sys[mid].item_s
+                  : WHICH == 3 ? sys[mid].item_f : sys[mid].item_t;
         if (first <= item) lo = mid; else hi = mid - 1;
     }
     return lo;
@@ -68,37 +69,48 @@ extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
 template <int K, bool ERR>
 __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int tile,
                                           const u32* __restrict__ blocks, const float4* __restrict__ quad,
+                                          const float4* __restrict__ tables, float4* tab_lds,
                                           float* __restrict__ model, float* __restrict__ errp)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
 #pragma unroll
     for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = i < d.dwords ? __builtin_amdgcn_readfirstlane(gb[i]) : 0u;
-    int k = tile * LENSED_RENDER_BS + (int)threadIdx.x;
-    if (k >= d.npix) return;
-    int py = k / d.width;
-    int px = k - py * d.width;
-    // pixel centre as kernel/lensed.cl:25, then one ray per node
-    const float cx = d.pcs[0] + d.pcs[2] * (float)px;
-    const float cy = d.pcs[1] + d.pcs[3] * (float)py;
-    const float4* q = quad + d.quad_off;
-    float f = 0.0f, e = 0.0f;
-    for (int n = 0; n < d.nq; ++n) {
-        const float4 node = q[n];
-        float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y));
-        f += node.z * v;
-        if (ERR) e += node.w * v;
+    const lcl::float4* tab = nullptr;
+    if (d.table_mode) {  // the candidate's EPL angular table, staged in LDS
+        const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
+        for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
+        __syncthreads();
+        tab = (const lcl::float4*)tab_lds;
     }
-    long long o = d.model_off + (long long)cand * d.npix + k;
-    model[o] = f;
-    if (ERR) errp[o] = e;
+    int k = tile * LENSED_RENDER_BS + (int)threadIdx.x;
+    if (k < d.npix) {
+        int py = k / d.width;
+        int px = k - py * d.width;
+        // pixel centre as kernel/lensed.cl:25, then one ray per node
+        const float cx = d.pcs[0] + d.pcs[2] * (float)px;
+        const float cy = d.pcs[1] + d.pcs[3] * (float)py;
+        const float4* q = quad + d.quad_off;
+        float f = 0.0f, e = 0.0f;
+        for (int n = 0; n < d.nq; ++n) {
+            const float4 node = q[n];
+            float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y), tab);
+            f += node.z * v;
+            if (ERR) e += node.w * v;
+        }
+        long long o = d.model_off + (long long)cand * d.npix + k;
+        model[o] = f;
+        if (ERR) errp[o] = e;
+    }
+    if (d.table_mode) __syncthreads();  // the table is replaced by the next item
 }
 
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
-    const u32* __restrict__ blocks, const float4* __restrict__ quad,
+    const u32* __restrict__ blocks, const float4* __restrict__ quad, const float4* __restrict__ tables,
     float* __restrict__ model, float* __restrict__ errp)
 {
+    extern __shared__ float4 tab_lds[];
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
         int s = lensed_find_sys<0>(sys, nsys, item);
         const lensed_desc& d = sys[s];
@@ -106,16 +118,73 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
         int cand = local / d.ntile_r;
         int tile = local - cand * d.ntile_r;
         switch (d.model) {
-#define LENSED_RENDER_CASE(K)                                                                    \
-    case K:                                                                                      \
-        if (errp) lensed_render_item<K, true>(d, cand, tile, blocks, quad, model, errp);         \
-        else lensed_render_item<K, false>(d, cand, tile, blocks, quad, model, errp);             \
+#define LENSED_RENDER_CASE(K)                                                                            \
+    case K:                                                                                              \
+        if (errp) lensed_render_item<K, true>(d, cand, tile, blocks, quad, tables, tab_lds, model, errp); \
+        else lensed_render_item<K, false>(d, cand, tile, blocks, quad, tables, tab_lds, model, errp);     \
         break;
             LENSED_FOR_MODELS(LENSED_RENDER_CASE)
 #undef LENSED_RENDER_CASE
             default: break;
         }
     }
+}
+
+// -------------------------------------------------------------- EPL tables
+// One thread per (candidate, octant, interval): the angular factor Omega and
+// dOmega/dtau of the tenth-order series at both ends of the interval, in
+// FP64 from the candidate's (q, t), turned into cubic Hermite coefficients.
+__device__ inline void lensed_epl_omega(double q, double t, double psi, double& ox, double& oy, double& dx,
+                                        double& dy)
+{
+    const double f = (1 - q) / (1 + q), T = 2 - t;
+    double c = 1.0;
+    ox = oy = dx = dy = 0.0;
+    for (int k = 0; k <= 10; ++k) {
+        if (k > 0) c *= -f * (2 * k - T) / (2 * k + T);
+        double m = 2 * k + 1, sn, cs;
+        sincos(m * psi, &sn, &cs);
+        ox += c * cs;
+        oy += c * sn;
+        dx -= c * m * sn;
+        dy += c * m * cs;
+    }
+}
+
+__device__ inline void lensed_epl_node(double q, double t, int half, double tau, double& ox, double& oy,
+                                       double& dx, double& dy)
+{
+    double a = atan(tau);
+    double psi = half ? 1.57079632679489661923 - a : a;
+    double gx, gy;
+    lensed_epl_omega(q, t, psi, ox, oy, gx, gy);
+    double dpsi = (half ? -1.0 : 1.0) / (1 + tau * tau) / LENSED_EPL_NT;  // d psi / d f
+    dx = gx * dpsi;
+    dy = gy * dpsi;
+}
+
+extern "C" __global__ __launch_bounds__(256) void lensed_epl_tables(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems,
+    const float* __restrict__ params, float4* __restrict__ tables)
+{
+    int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    int s = lensed_find_sys<4>(sys, nsys, item);
+    const lensed_desc& d = sys[s];
+    int local = item - d.item_t;
+    int cand = local / (2 * LENSED_EPL_NT);
+    int rem = local - cand * 2 * LENSED_EPL_NT;
+    int half = rem / LENSED_EPL_NT, i = rem - half * LENSED_EPL_NT;
+    const float* p = params + d.param_off + (long long)cand * d.npar;
+    double q = p[d.epl_q], t = p[d.epl_t];
+    double x0, y0, dx0, dy0, x1, y1, dx1, dy1;
+    lensed_epl_node(q, t, half, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
+    lensed_epl_node(q, t, half, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
+    float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4 + 2 * rem;
+    out[0] = make_float4((float)x0, (float)dx0, (float)(3 * (x1 - x0) - 2 * dx0 - dx1),
+                         (float)(2 * (x0 - x1) + dx0 + dx1));
+    out[1] = make_float4((float)y0, (float)dy0, (float)(3 * (y1 - y0) - 2 * dy0 - dy1),
+                         (float)(2 * (y0 - y1) + dy0 + dy1));
 }
 
 // --------------------------------------------------------------- conv+chi2

@@ -23,7 +23,7 @@ extern "C" int host_render(const float* params, int W, int H, const float* pcs, 
         float cx = pcs[0] + pcs[2] * (float)(k % W), cy = pcs[1] + pcs[3] * (float)(k / W);
         float f = 0, e = 0;
         for (int n = 0; n < nq; ++n) {
-            float v = lcl::compute_0(data, lcl::float2(cx + qq[2 * n], cy + qq[2 * n + 1]));
+            float v = lcl::compute_0(data, lcl::float2(cx + qq[2 * n], cy + qq[2 * n + 1]), nullptr);
             f += ww[2 * n] * v;
             e += ww[2 * n + 1] * v;
         }

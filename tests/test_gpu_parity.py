@@ -300,3 +300,37 @@ def test_c3_full_size_batch_properties(c3):
         planes = ctx.render(truth)
         rel, _ = rel_err(planes["image"], model)
         assert rel <= 1e-4, rel
+
+
+def near_truth_phys(wl, pids, n, scale, seed=0):
+    """Parameter vectors scattered around the truth (object order)."""
+    rng = np.random.default_rng(seed)
+    truth = np.array([wl.truth[k] for k in pids])
+    out = truth + scale * rng.normal(size=(n, len(pids))) * np.maximum(np.abs(truth), 1.0)
+    qi = pids.index("lens.q")
+    out[:, qi] = np.clip(out[:, qi], 0.3, 1.0)
+    return out
+
+
+def test_c3_epl_table_matches_exact(c3):
+    """The tabulated EPL angular factor against the per-ray series and the
+    FP64 oracle: model images and logL near the truth."""
+    from lensed_amd._lib import LENSED_TABLE
+    wl, pids = c3
+    phys = near_truth_phys(wl, pids, 64, 2e-4)
+    cubes = wl.candidates(14, 256)
+    with Context(wl.spec(), max_batch=256) as ex:
+        le = ex.loglike_phys(phys)
+        lc, _ = ex.loglike_batch(cubes)
+    with Context(wl.spec(), max_batch=256, flags=LENSED_TABLE | LENSED_GRAPH) as tb:
+        lt = tb.loglike_phys(phys)
+        ltc, _ = tb.loglike_batch(cubes)
+        single = tb.loglike(cubes[3])[0]
+        truth = np.array([wl.truth[k] for k in pids])
+        planes = tb.render(truth)
+    assert np.max(np.abs(lt - le)) <= 1e-2, np.max(np.abs(lt - le))
+    assert np.all(np.abs(ltc - lc) <= 1e-2 + 2e-5 * np.abs(lc))
+    assert single == ltc[3]
+    model = oracle_model(wl.objects, truth, wl.shape, wl.psf)
+    rel, _ = rel_err(planes["image"], model)
+    assert rel <= 1e-4, rel
