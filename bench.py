@@ -57,9 +57,16 @@ def make_workload(K):
     return wl, pids
 
 
+EMULATOR = os.path.join(ROOT, "lensed_amd", "data", "epl_mlp.bin")
+
+
 def time_batched(wl, K, steps, warmup, flags):
+    """Seconds for `steps` launches of K candidates; logL of the last; the
+    emulator's statistics when the emulated path runs."""
+    from lensed_amd._lib import LENSED_EMULATE
     from lensed_amd.lensed import Context
-    with Context(wl.spec(), max_batch=K, flags=flags) as ctx:
+    emu = EMULATOR if flags & LENSED_EMULATE else None
+    with Context(wl.spec(), max_batch=K, flags=flags, emulator=emu) as ctx:
         cubes = wl.candidates(ctx.ndims(), K)
         for _ in range(warmup):
             ctx.loglike_batch(cubes)
@@ -67,7 +74,8 @@ def time_batched(wl, K, steps, warmup, flags):
         for _ in range(steps):
             lnew, _ = ctx.loglike_batch(cubes)
         dt = time.perf_counter() - t0
-    return dt, lnew
+        stats = ctx.emulator_stats() if emu else None
+    return dt, lnew, stats
 
 
 def time_plain(wl, n):
@@ -96,9 +104,10 @@ def time_plain(wl, n):
 
 
 def kernel_times(wl, K, launches, flags=0):
-    from lensed_amd._lib import LENSED_PROFILE
+    from lensed_amd._lib import LENSED_EMULATE, LENSED_PROFILE
     from lensed_amd.lensed import Context
-    with Context(wl.spec(), max_batch=K, flags=LENSED_PROFILE | flags) as ctx:
+    emu = EMULATOR if flags & LENSED_EMULATE else None
+    with Context(wl.spec(), max_batch=K, flags=LENSED_PROFILE | flags, emulator=emu) as ctx:
         cubes = wl.candidates(ctx.ndims(), K)
         ctx.loglike_batch(cubes)
         calls0, ms0 = ctx.profile()
@@ -123,13 +132,15 @@ def near_truth(wl, pids, n, scale=2e-4, seed=7):
 def accuracy(wl, pids, K, flags):
     """|delta logL| of a path against the exact per-ray path on the same
     candidates: near the truth, and relative over the prior draws."""
+    from lensed_amd._lib import LENSED_EMULATE
     from lensed_amd.lensed import Context
+    emu = EMULATOR if flags & LENSED_EMULATE else None
     phys = near_truth(wl, pids, 512)
     with Context(wl.spec(), max_batch=K) as ex:
         cubes = wl.candidates(ex.ndims(), K)
         le = ex.loglike_phys(phys)
         lc, _ = ex.loglike_batch(cubes)
-    with Context(wl.spec(), max_batch=K, flags=flags) as c:
+    with Context(wl.spec(), max_batch=K, flags=flags, emulator=emu) as c:
         lo = c.loglike_phys(phys)
         loc, _ = c.loglike_batch(cubes)
     return {"max_abs_dlogL_near_truth": float(np.max(np.abs(lo - le))),
@@ -178,19 +189,22 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from lensed_amd._lib import LENSED_GRAPH, LENSED_TABLE
+    from lensed_amd._lib import LENSED_EMULATE, LENSED_GRAPH, LENSED_TABLE
     K = args.batch
     wl, pids = make_workload(K)
 
-    dt_exact, lnew = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH)
+    dt_exact, lnew, _ = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH)
     exact_rate = args.steps * K / dt_exact
-    dt, _ = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_TABLE)
+    dt_table, _, _ = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_TABLE)
+    table_rate = args.steps * K / dt_table
+    dt, _, emu_stats = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_EMULATE)
     value = args.steps * K / dt
-    acc = accuracy(wl, pids, K, LENSED_TABLE)
+    acc = accuracy(wl, pids, K, LENSED_EMULATE)
+    acc_table = accuracy(wl, pids, K, LENSED_TABLE)
     plain_rate, lplain = time_plain(wl, args.plain)
     # plain and batched use the same kernels: identical logL per candidate
     same = bool(np.array_equal(lplain, lnew[: len(lplain)]))
-    kt = kernel_times(wl, K, 3, LENSED_TABLE)
+    kt = kernel_times(wl, K, 3, LENSED_EMULATE)
     kt_exact = kernel_times(wl, K, 2)
     render_s = kt["render"] / 1e3
     achieved = FLOP_RENDER * K / render_s / 1e12
@@ -213,8 +227,17 @@ def main():
         "config": {"workload": "C3: 256x256 px, epl_plus_shear + gauss (sigma 2 px), 21x21 Gaussian PSF "
                                "FWHM 3 px, g3k7 49 rays/px, sigma 0.05, candidates uniform from the priors",
                    "candidates_per_launch": K,
-                   "path": "batched, EPL angular factor from the per-candidate FP64-built table, hipGraph replay",
+                   "path": "batched + emulated: EPL angular-factor tables from the BF16-MFMA emulator, checked "
+                           "per candidate against the FP64 series at every interval midpoint, exact-series "
+                           "fallback where the error exceeds 4e-6; hipGraph replay",
                    "seeds": {"noise": wl.seed, "candidates": wl.cand_seed}},
+        "emulator": {"fallbacks": emu_stats[0], "checked": emu_stats[1],
+                     "max_heldout_abs_error_vs_fp64_series": emu_stats[2], "bound": 4e-6,
+                     "weights": "lensed_amd/data/epl_mlp.bin (2^22 seeded samples, 40 epochs, trained on the "
+                                "GPU by lensed_amd.train_emulator)"},
+        "batched_table": {"value": round(table_rate, 2), "unit": "logL evals/s",
+                          "path": "batched, EPL angular factor from the per-candidate FP64-built table",
+                          "accuracy_vs_exact": acc_table},
         "batched_exact": {"value": round(exact_rate, 2), "unit": "logL evals/s",
                           "path": "batched, per-ray tenth-order EPL series"},
         "accuracy_vs_exact": acc,
@@ -228,6 +251,7 @@ def main():
                      "traffic": None, "algorithmic_flop_per_launch": FLOP_RENDER * K,
                      "avg_launch_ms": round(kt["render"], 4)},
         "stages_ms_per_launch": {k: round(v, 4) for k, v in kt.items()},
+        "stages_note": "set_params includes the emulator, held-out check and fallback table kernels",
         "stages_ms_per_launch_exact": {k: round(v, 4) for k, v in kt_exact.items()},
         "conv_chi2": {"achieved_tflops": round((FLOP_CONV + FLOP_CHI2) * K / conv_s / 1e12, 3),
                       "avg_launch_ms": round(kt["convolve"], 4)},

@@ -153,6 +153,7 @@ typedef struct {
     int flags;            /* LENSED_* */
     size_t max_batch;     /* largest number of candidates per call */
     const char* emulator; /* weights file for LENSED_EMULATE, or NULL */
+    double emulator_tol;  /* held-out bound on the emulated angular factor, 0 = 4e-6 */
 } lensed_options;
 
 /* Build the context for one or more systems (several systems are evaluated
@@ -204,6 +205,27 @@ int lensed_render(lensed* l, size_t s, const double* phys, float* value, float* 
  * +params, -params, set_params, render, convolve, loglike, +loglike_mem,
  * -loglike_mem (src/lensed.c:1125-1132); returns the number of calls */
 long lensed_profile(const lensed* l, double* ms8);
+
+/* --------------------------------------------------------- EPL emulator */
+
+/* No counterpart in the reference: the learned stand-in for the EPL
+ * angular factor (objects/epl.cl:50-61 series).  MLP 4 -> 3 x 128 SiLU -> 2
+ * trained on the GPU on nsamples (a power of two) seeded samples of the
+ * series; writes the weights file; stats[0] = final training MSE,
+ * stats[1] = seconds. */
+int lensed_emulator_train(const char* path, long long nsamples, int epochs, int batch, double lr,
+                          unsigned long long seed, double* stats);
+/* read a weights file (kParams floats); 0 on success */
+int lensed_emulator_read(const char* path, float* weights);
+/* FP32 loss (mean squared error) and gradient of the training kernels */
+double lensed_emulator_loss_grad(const float* weights, const float* X, const float* Y, int n, float* grad);
+/* the network at X[n][4] with tangents dX[n][4]; bf16 = 1 runs the
+ * BF16-MFMA inference kernel (outputs Y[n][2] and dY[n][2]) */
+int lensed_emulator_eval(const float* weights, const float* X, const float* dX, int n, int bf16, float* Y,
+                         float* dY);
+/* candidates of the last emulated launch whose held-out check failed (and
+ * fell back to the exact series), candidates checked, largest held-out error */
+int lensed_emulator_stats(const lensed* l, long* fallbacks, long* checked, double* max_err);
 
 /* results recorded by dumper(): mean, sigma, ML, MAP per parameter in object
  * order (each may be NULL) and the log-evidence summary */

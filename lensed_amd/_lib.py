@@ -32,7 +32,8 @@ class System(Structure):
 
 
 class Options(Structure):
-    _fields_ = [("device", c_int), ("flags", c_int), ("max_batch", c_size_t), ("emulator", c_char_p)]
+    _fields_ = [("device", c_int), ("flags", c_int), ("max_batch", c_size_t), ("emulator", c_char_p),
+                ("emulator_tol", c_double)]
 
 
 # every entry point declared in include/lensed.h: name -> (restype, argtypes)
@@ -74,6 +75,14 @@ SIGNATURES = {
     "lensed_render": (c_int, [c_void_p, c_size_t, POINTER(c_double), POINTER(c_float), POINTER(c_float),
                               POINTER(c_float), POINTER(c_float)]),
     "lensed_profile": (c_long, [c_void_p, POINTER(c_double)]),
+    "lensed_emulator_train": (c_int, [c_char_p, ctypes.c_longlong, c_int, c_int, c_double, ctypes.c_ulonglong,
+                                      POINTER(c_double)]),
+    "lensed_emulator_read": (c_int, [c_char_p, POINTER(c_float)]),
+    "lensed_emulator_loss_grad": (c_double, [POINTER(c_float), POINTER(c_float), POINTER(c_float), c_int,
+                                             POINTER(c_float)]),
+    "lensed_emulator_eval": (c_int, [POINTER(c_float), POINTER(c_float), POINTER(c_float), c_int, c_int,
+                                     POINTER(c_float), POINTER(c_float)]),
+    "lensed_emulator_stats": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
     "lensed_results": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double),
                                POINTER(c_double), POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
 }

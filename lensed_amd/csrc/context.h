@@ -60,6 +60,7 @@ struct Layout {
     std::vector<lensed_desc> desc;  // systems with ncand > 0
     std::vector<int> sys_of_desc;
     int n_set = 0, n_render = 0, n_conv = 0, n_final = 0, n_table = 0;
+    int n_nodes = 0, n_tcands = 0;  // emulator: table nodes and candidates with a table
     size_t lds_conv = 0, lds_render = 0;
     bool conv_small = false;        // short tiles: launch too small to fill the GPU
     lensed_desc* d_desc = nullptr;
@@ -93,6 +94,14 @@ struct lensed {
     double* d_partials = nullptr;
     double* d_logl = nullptr;
     void* d_tables = nullptr;   // EPL angular tables, float4 [candidates][LENSED_EPL_TABLE_F4]
+    // EPL emulator: weights (fp32 and packed bf16), table nodes, fallback flags
+    void* emu_wf = nullptr;
+    void* emu_wp = nullptr;
+    void* d_nodes = nullptr;
+    int* d_flags = nullptr;
+    float* d_errs = nullptr;
+    float emu_tol = 4e-6f;
+    const lsd::Layout* last_table_layout = nullptr;
     // pinned host staging
     float* h_params = nullptr;
     double* h_logl = nullptr;

@@ -1,0 +1,738 @@
+// EPL emulator kernels: training data from the exact series, FP32 training
+// (forward, backward, Adam), BF16-MFMA inference of the angular-factor table
+// nodes with FP32 accumulation, and the held-out check that decides the
+// exact fallback per candidate.  Network and weight layout: emulator.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/lensed.h"
+#include "../kernels/desc.h"
+#include "../kernels/epl_series.h"
+#include "emulator.h"
+
+namespace lsd {
+void set_error(const std::string& msg);
+}
+
+namespace lsd {
+namespace emu {
+
+#define EMU_HIP(x)                                                             \
+    do {                                                                       \
+        hipError_t e_ = (x);                                                   \
+        if (e_ != hipSuccess) {                                                \
+            set_error(std::string(#x) + ": " + hipGetErrorString(e_));         \
+            return -1;                                                         \
+        }                                                                      \
+    } while (0)
+
+// ------------------------------------------------------------------ data
+
+__device__ inline unsigned long long mix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+static inline unsigned long long mix64_host(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ inline double uniform01(unsigned long long seed, long long i, int dim) {
+    return (double)(mix64(mix64(seed) ^ (unsigned long long)(i * 4 + dim)) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// seeded samples: q ~ U[0.3, 1], t ~ U[0.5, 1.5], psi ~ U[0, pi/2]; target
+// Omega - (cos psi, sin psi) from the FP64 series
+__global__ void k_make_data(long long n, unsigned long long seed, float* __restrict__ X, float* __restrict__ Y) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double q = 0.3 + 0.7 * uniform01(seed, i, 0);
+    double t = 0.5 + uniform01(seed, i, 1);
+    double psi = 1.57079632679489661923 * uniform01(seed, i, 2);
+    double ox, oy, dx, dy;
+    lensed_epl_omega(q, t, psi, ox, oy, dx, dy);
+    double c = cos(psi), s = sin(psi);
+    X[4 * i] = (float)((q - kQ0) / kQS);
+    X[4 * i + 1] = (float)((t - kT0) / kTS);
+    X[4 * i + 2] = (float)c;
+    X[4 * i + 3] = (float)s;
+    Y[2 * i] = (float)(ox - c);
+    Y[2 * i + 1] = (float)(oy - s);
+}
+
+// minibatch b of an epoch: sample (start + i * stride) mod n, stride odd so
+// every epoch visits each sample once (n a power of two)
+__global__ void k_gather(int rows, long long n, long long start, long long stride, const float* __restrict__ X,
+                         const float* __restrict__ Y, float* __restrict__ xb, float* __restrict__ yb) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    long long j = (start + (long long)i * stride) & (n - 1);
+    for (int k = 0; k < kIn; ++k) xb[i * kIn + k] = X[j * kIn + k];
+    for (int k = 0; k < kOut; ++k) yb[i * kOut + k] = Y[j * kOut + k];
+}
+
+// ------------------------------------------------------------- FP32 GEMM
+// C[M][N] (+)= sum_k A(m, k) B(k, n), A and B addressed by strides, 64 x 64
+// tiles, 4 x 4 outputs per thread, K staged in LDS in steps of 16.
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* __restrict__ A, long long sAm,
+                                              long long sAk, const float* __restrict__ B, long long sBk,
+                                              long long sBn, float* __restrict__ C, long long ldc, int accumulate) {
+    __shared__ float As[16][64 + 1];
+    __shared__ float Bs[16][64 + 1];
+    const int tm = threadIdx.x / 16, tn = threadIdx.x % 16;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    float acc[4][4] = {};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        for (int i = threadIdx.x; i < 64 * 16; i += 256) {
+            int mm = i / 16, kk = i % 16;
+            int gm = m0 + mm, gk = k0 + kk;
+            As[kk][mm] = (gm < M && gk < K) ? A[gm * sAm + gk * sAk] : 0.0f;
+        }
+        for (int i = threadIdx.x; i < 16 * 64; i += 256) {
+            int kk = i / 64, nn = i % 64;
+            int gk = k0 + kk, gn = n0 + nn;
+            Bs[kk][nn] = (gk < K && gn < N) ? B[gk * sBk + gn * sBn] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][tm * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tn * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int gm = m0 + tm * 4 + i, gn = n0 + tn * 4 + j;
+            if (gm < M && gn < N) {
+                float* c = C + gm * ldc + gn;
+                *c = accumulate ? *c + acc[i][j] : acc[i][j];
+            }
+        }
+}
+
+__device__ inline float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+// Z += bias; H = silu(Z)
+__global__ void k_bias_silu(int rows, int cols, float* __restrict__ Z, const float* __restrict__ b,
+                            float* __restrict__ H) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * cols) return;
+    float z = Z[i] + b[i % cols];
+    Z[i] = z;
+    H[i] = z * sigmoid(z);
+}
+
+// output layer: pred = Z + b; dZ = d(mean squared error)/dZ; loss partials
+__global__ void k_out_grad(int rows, float* __restrict__ Z, const float* __restrict__ b, const float* __restrict__ Y,
+                           float* __restrict__ dZ, double* __restrict__ loss) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (i < rows * kOut) {
+        float p = Z[i] + b[i % kOut];
+        Z[i] = p;
+        float d = p - Y[i];
+        dZ[i] = 2.0f * d / (float)(rows * kOut);
+        l = (double)d * d;
+    }
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
+    if ((threadIdx.x & 63) == 0 && loss) atomicAdd(loss, l);
+}
+
+// dZ = dH * silu'(Z), in place on dH
+__global__ void k_silu_back(int n, float* __restrict__ dH, const float* __restrict__ Z) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float z = Z[i], s = sigmoid(z);
+    dH[i] *= s * (1.0f + z * (1.0f - s));
+}
+
+// db[c] = sum over rows of dZ[r][c]
+__global__ void k_colsum(int rows, int cols, const float* __restrict__ dZ, float* __restrict__ db) {
+    int c = blockIdx.x;
+    float s = 0.0f;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) s += dZ[r * cols + c];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __shared__ float part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) db[c] = part[0] + part[1] + part[2] + part[3];
+}
+
+__global__ void k_adam(int n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1 - b1) * gi;
+    float vi = b2 * v[i] + (1 - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Workspace of one forward/backward pass over `rows` samples.
+struct Net {
+    int rows = 0;
+    float *w = nullptr, *g = nullptr;
+    float *xb = nullptr, *yb = nullptr;
+    float *z1 = nullptr, *h1 = nullptr, *z2 = nullptr, *h2 = nullptr, *z3 = nullptr, *h3 = nullptr, *z4 = nullptr;
+    float *d4 = nullptr, *dh = nullptr, *dh2 = nullptr;
+    double* loss = nullptr;
+    int alloc(int r) {
+        rows = r;
+        size_t H = (size_t)r * kHidden;
+        EMU_HIP(hipMalloc(&w, kParams * 4));
+        EMU_HIP(hipMalloc(&g, kParams * 4));
+        EMU_HIP(hipMalloc(&xb, (size_t)r * kIn * 4));
+        EMU_HIP(hipMalloc(&yb, (size_t)r * kOut * 4));
+        for (float** p : {&z1, &h1, &z2, &h2, &z3, &h3, &dh, &dh2}) EMU_HIP(hipMalloc(p, H * 4));
+        EMU_HIP(hipMalloc(&z4, (size_t)r * kOut * 4));
+        EMU_HIP(hipMalloc(&d4, (size_t)r * kOut * 4));
+        EMU_HIP(hipMalloc(&loss, 8));
+        return 0;
+    }
+    void release() {
+        for (void* p : {(void*)w, (void*)g, (void*)xb, (void*)yb, (void*)z1, (void*)h1, (void*)z2, (void*)h2,
+                        (void*)z3, (void*)h3, (void*)z4, (void*)d4, (void*)dh, (void*)dh2, (void*)loss})
+            if (p) (void)hipFree(p);
+    }
+};
+
+static void gemm(hipStream_t st, int M, int N, int K, const float* A, long long sAm, long long sAk, const float* B,
+                 long long sBk, long long sBn, float* C, long long ldc, int acc) {
+    dim3 grid((N + 63) / 64, (M + 63) / 64);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, st, M, N, K, A, sAm, sAk, B, sBk, sBn, C, ldc, acc);
+}
+
+static inline dim3 blocks(long long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// forward pass over net.xb; z4 = prediction without bias until k_out_grad
+static void forward(hipStream_t st, Net& net) {
+    const int R = net.rows, Hd = kHidden;
+    const float* w = net.w;
+    // layer l: Z[R][out] = X[R][in] W^T  (W row-major [out][in])
+    gemm(st, R, Hd, kIn, net.xb, kIn, 1, w + kW1, 1, kIn, net.z1, Hd, 0);
+    hipLaunchKernelGGL(k_bias_silu, blocks((long long)R * Hd), dim3(256), 0, st, R, Hd, net.z1, w + kB1, net.h1);
+    gemm(st, R, Hd, Hd, net.h1, Hd, 1, w + kW2, 1, Hd, net.z2, Hd, 0);
+    hipLaunchKernelGGL(k_bias_silu, blocks((long long)R * Hd), dim3(256), 0, st, R, Hd, net.z2, w + kB2, net.h2);
+    gemm(st, R, Hd, Hd, net.h2, Hd, 1, w + kW3, 1, Hd, net.z3, Hd, 0);
+    hipLaunchKernelGGL(k_bias_silu, blocks((long long)R * Hd), dim3(256), 0, st, R, Hd, net.z3, w + kB3, net.h3);
+    gemm(st, R, kOut, Hd, net.h3, Hd, 1, w + kW4, 1, Hd, net.z4, kOut, 0);
+}
+
+// backward pass after k_out_grad filled d4; gradients into net.g
+static void backward(hipStream_t st, Net& net) {
+    const int R = net.rows, Hd = kHidden;
+    const float* w = net.w;
+    float* g = net.g;
+    // layer 4: dW4 = d4^T h3, db4, dh3 = d4 W4
+    gemm(st, kOut, Hd, R, net.d4, 1, kOut, net.h3, Hd, 1, g + kW4, Hd, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(kOut), dim3(256), 0, st, R, kOut, net.d4, g + kB4);
+    gemm(st, R, Hd, kOut, net.d4, kOut, 1, w + kW4, Hd, 1, net.dh, Hd, 0);
+    hipLaunchKernelGGL(k_silu_back, blocks((long long)R * Hd), dim3(256), 0, st, R * Hd, net.dh, net.z3);
+    // layer 3
+    gemm(st, Hd, Hd, R, net.dh, 1, Hd, net.h2, Hd, 1, g + kW3, Hd, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(Hd), dim3(256), 0, st, R, Hd, net.dh, g + kB3);
+    gemm(st, R, Hd, Hd, net.dh, Hd, 1, w + kW3, Hd, 1, net.dh2, Hd, 0);
+    hipLaunchKernelGGL(k_silu_back, blocks((long long)R * Hd), dim3(256), 0, st, R * Hd, net.dh2, net.z2);
+    // layer 2
+    gemm(st, Hd, Hd, R, net.dh2, 1, Hd, net.h1, Hd, 1, g + kW2, Hd, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(Hd), dim3(256), 0, st, R, Hd, net.dh2, g + kB2);
+    gemm(st, R, Hd, Hd, net.dh2, Hd, 1, w + kW2, Hd, 1, net.dh, Hd, 0);
+    hipLaunchKernelGGL(k_silu_back, blocks((long long)R * Hd), dim3(256), 0, st, R * Hd, net.dh, net.z1);
+    // layer 1
+    gemm(st, Hd, kIn, R, net.dh, 1, Hd, net.xb, kIn, 1, g + kW1, kIn, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(Hd), dim3(256), 0, st, R, Hd, net.dh, g + kB1);
+}
+
+// ------------------------------------------------------ BF16 MFMA inference
+// v_mfma_f32_32x32x8bf16_1k, wave64: A (32 x 8) lane l holds A[l%32][4(l/32)+0..3],
+// B (8 x 32) lane l holds B[4(l/32)+0..3][l%32], D (32 x 32) lane l register v
+// holds D[8(v/4) + 4(l/32) + v%4][l%32].  A layer computes Z^T = W H^T for 32
+// samples per wave: M = 128 outputs in 4 tiles, K = 128 in 16 steps.  The
+// accumulator of output tile mt holds exactly the B operand of k-steps
+// 4 mt .. 4 mt + 3 of the next layer, so activations never leave registers.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline short f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (short)(u >> 16);
+}
+
+__device__ inline float silu_d(float z, float& dsilu) {
+    float s = 1.0f / (1.0f + __expf(-z));
+    dsilu = s * (1.0f + z * (1.0f - s));
+    return z * s;
+}
+
+// one hidden 128 x 128 layer in MFMA, primal and tangent (no bias on the tangent)
+__device__ inline void mfma_layer(const bf16x4* __restrict__ wl, const float* __restrict__ bias, const bf16x4 (&hb)[16],
+                                  const bf16x4 (&tb)[16], float (&z)[4][16], float (&dz)[4][16], int lane) {
+    f32x16 acc[4], tac[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[mt][v] = tac[mt][v] = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            bf16x4 a = wl[(mt * 16 + s) * 64 + lane];
+            acc[mt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, hb[s], acc[mt], 0, 0, 0);
+            tac[mt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, tb[s], tac[mt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            int o = 32 * mt + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+            z[mt][v] = acc[mt][v] + bias[o];
+            dz[mt][v] = tac[mt][v];
+        }
+}
+
+// activations (z -> silu) of a layer as the bf16 B operands of the next,
+// with the tangent silu'(z) dz
+__device__ inline void to_operands(const float (&z)[4][16], const float (&dz)[4][16], bf16x4 (&hb)[16],
+                                   bf16x4 (&tb)[16], float (&h)[4][16], float (&dh)[4][16]) {
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            float ds;
+            h[mt][v] = silu_d(z[mt][v], ds);
+            dh[mt][v] = ds * dz[mt][v];
+        }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int gq = 0; gq < 4; ++gq) {
+            bf16x4 a, b;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = f2bf(h[mt][4 * gq + i]);
+                b[i] = f2bf(dh[mt][4 * gq + i]);
+            }
+            hb[4 * mt + gq] = a;
+            tb[4 * mt + gq] = b;
+        }
+}
+
+// the network on 32 samples per wave (lanes l and l + 32 share sample l % 32):
+// output y and its derivative dy along the input direction dx
+__device__ inline void mlp_wave(const float* __restrict__ ws, const bf16x4* __restrict__ w2, const bf16x4* __restrict__ w3,
+                                const float (&x)[4], const float (&dx)[4], float (&y)[2], float (&dy)[2], int lane) {
+    float z[4][16], dz[4][16], h[4][16], dh[4][16];
+    // layer 1 in FP32 (K = 4)
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            int o = 32 * mt + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+            const float* wr = ws + kW1 + o * kIn;
+            z[mt][v] = ws[kB1 + o] + wr[0] * x[0] + wr[1] * x[1] + wr[2] * x[2] + wr[3] * x[3];
+            dz[mt][v] = wr[0] * dx[0] + wr[1] * dx[1] + wr[2] * dx[2] + wr[3] * dx[3];
+        }
+    bf16x4 hb[16], tb[16];
+    to_operands(z, dz, hb, tb, h, dh);
+    mfma_layer(w2, ws + kB2, hb, tb, z, dz, lane);
+    to_operands(z, dz, hb, tb, h, dh);
+    mfma_layer(w3, ws + kB3, hb, tb, z, dz, lane);
+    to_operands(z, dz, hb, tb, h, dh);
+    // layer 4 in FP32: each lane holds 64 of the 128 inputs of its sample
+#pragma unroll
+    for (int c = 0; c < kOut; ++c) {
+        float acc = 0.0f, tac = 0.0f;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                int o = 32 * mt + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+                float wv = ws[kW4 + c * kHidden + o];
+                acc += wv * h[mt][v];
+                tac += wv * dh[mt][v];
+            }
+        acc += __shfl_xor(acc, 32, 64);
+        tac += __shfl_xor(tac, 32, 64);
+        y[c] = acc + ws[kB4 + c];
+        dy[c] = tac;
+    }
+}
+
+// raw inference for tests: X[n][4], dX[n][4] -> Y[n][2], dY[n][2]
+__global__ __launch_bounds__(256) void k_mlp_raw(int n, const float* __restrict__ wf, const bf16x4* __restrict__ wp,
+                                                 const float* __restrict__ X, const float* __restrict__ dX,
+                                                 float* __restrict__ Y, float* __restrict__ dY) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int i = wave * 32 + (lane & 31);
+    const int ic = min(i, n - 1);
+    float x[4], dx[4], y[2], dy[2];
+    for (int k = 0; k < 4; ++k) {
+        x[k] = X[ic * 4 + k];
+        dx[k] = dX[ic * 4 + k];
+    }
+    mlp_wave(wf, wp, wp + 4 * 16 * 64, x, dx, y, dy, lane);
+    if (lane < 32 && i < n) {
+        Y[2 * i] = y[0];
+        Y[2 * i + 1] = y[1];
+        dY[2 * i] = dy[0];
+        dY[2 * i + 1] = dy[1];
+    }
+}
+
+// table nodes: one sample per (candidate, octant, node j = 0..NT); the node
+// value Omega and its f-derivative, from the network
+template <int WHICH>
+__device__ inline int find_sys_t(const lensed_desc* __restrict__ sys, int nsys, int item) {
+    int lo = 0, hi = nsys - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        int first = WHICH == 0 ? sys[mid].item_t : sys[mid].item_e;
+        if (first <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+constexpr int kNodes = 2 * (LENSED_EPL_NT + 1);  // per candidate
+
+__global__ __launch_bounds__(256) void k_emu_nodes(const lensed_desc* __restrict__ sys, int nsys, int nitems,
+                                                   const float* __restrict__ params, const float* __restrict__ wf,
+                                                   const bf16x4* __restrict__ wp, float4* __restrict__ nodes) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int g = wave * 32 + (lane & 31);
+    const int gc = min(g, nitems - 1);
+    const int s = find_sys_t<1>(sys, nsys, gc);
+    const lensed_desc& d = sys[s];
+    const int local = gc - d.item_e;
+    const int cand = local / kNodes;
+    const int rem = local - cand * kNodes;
+    const int half = rem / (LENSED_EPL_NT + 1), j = rem - half * (LENSED_EPL_NT + 1);
+    const float* p = params + d.param_off + (long long)cand * d.npar;
+    const float q = p[d.epl_q], t = p[d.epl_t];
+    const float tau = (float)j / LENSED_EPL_NT;
+    const float at = atanf(tau);
+    const float psi = half ? 1.57079632679489661923f - at : at;
+    const float dpsi = (half ? -1.0f : 1.0f) / (1.0f + tau * tau) / LENSED_EPL_NT;  // d psi / d f
+    float sn, cs;
+    sincosf(psi, &sn, &cs);
+    float x[4] = {(q - kQ0) / kQS, (t - kT0) / kTS, cs, sn};
+    float dx[4] = {0.0f, 0.0f, -sn * dpsi, cs * dpsi};
+    float y[2], dy[2];
+    mlp_wave(wf, wp, wp + 4 * 16 * 64, x, dx, y, dy, lane);
+    if (lane < 32 && g < nitems) {
+        long long cb = d.table_off / LENSED_EPL_TABLE_F4 + cand;
+        nodes[cb * kNodes + rem] = make_float4(cs + y[0], sn + y[1], dx[2] + dy[0], dx[3] + dy[1]);
+    }
+}
+
+// nodes -> cubic Hermite coefficients per interval (the table layout of
+// epl_table.h)
+__global__ __launch_bounds__(256) void k_emu_assemble(const lensed_desc* __restrict__ sys, int nsys, int nitems,
+                                                      const float4* __restrict__ nodes, float4* __restrict__ tables) {
+    int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    int s = find_sys_t<0>(sys, nsys, item);
+    const lensed_desc& d = sys[s];
+    int local = item - d.item_t;
+    int cand = local / (2 * LENSED_EPL_NT);
+    int rem = local - cand * 2 * LENSED_EPL_NT;
+    int half = rem / LENSED_EPL_NT, i = rem - half * LENSED_EPL_NT;
+    long long cb = d.table_off / LENSED_EPL_TABLE_F4 + cand;
+    float4 a = nodes[cb * kNodes + half * (LENSED_EPL_NT + 1) + i];
+    float4 b = nodes[cb * kNodes + half * (LENSED_EPL_NT + 1) + i + 1];
+    float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4 + 2 * rem;
+    out[0] = lensed_hermite(a.x, a.z, b.x, b.z);
+    out[1] = lensed_hermite(a.y, a.w, b.y, b.w);
+}
+
+// held-out check, one wave per candidate: the table against the FP64 series
+// at every interval midpoint (never a training or node point); flag the
+// candidate for the exact fallback if the largest error exceeds eps
+__global__ __launch_bounds__(256) void k_emu_check(const lensed_desc* __restrict__ sys, int nsys, int ncands,
+                                                   const float* __restrict__ params, const float4* __restrict__ tables,
+                                                   float eps, int* __restrict__ flags, float* __restrict__ errs) {
+    int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (w >= ncands) return;
+    // candidate w of the launch: walk the systems (few) to find its own
+    int s = 0;
+    for (; s < nsys - 1; ++s)
+        if (w < sys[s + 1].item_e / kNodes) break;
+    const lensed_desc& d = sys[s];
+    int cand = w - d.item_e / kNodes;
+    const float* p = params + d.param_off + (long long)cand * d.npar;
+    double q = p[d.epl_q], t = p[d.epl_t];
+    const float4* tab = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
+    float err = 0.0f;
+    for (int m = lane; m < 2 * LENSED_EPL_NT; m += 64) {
+        int half = m / LENSED_EPL_NT, i = m - half * LENSED_EPL_NT;
+        double ox, oy, dx, dy;
+        lensed_epl_node(q, t, half, (i + 0.5) / LENSED_EPL_NT, ox, oy, dx, dy);
+        float4 cx = tab[2 * m], cy = tab[2 * m + 1];
+        float f = 0.5f;
+        float tx = cx.x + f * (cx.y + f * (cx.z + f * cx.w));
+        float ty = cy.x + f * (cy.y + f * (cy.z + f * cy.w));
+        err = fmaxf(err, fmaxf(fabsf(tx - (float)ox), fabsf(ty - (float)oy)));
+    }
+    for (int o = 32; o > 0; o >>= 1) err = fmaxf(err, __shfl_xor(err, o, 64));
+    if (lane == 0) {
+        long long cb = d.table_off / LENSED_EPL_TABLE_F4 + cand;
+        flags[cb] = err > eps ? 1 : 0;
+        if (errs) errs[cb] = err;
+    }
+}
+
+// host: bf16 copies of W2, W3 in MFMA operand order
+static std::vector<unsigned short> pack_bf16(const float* w) {
+    std::vector<unsigned short> out(2 * 4 * 16 * 64 * 4);
+    auto bf = [](float f) {
+        unsigned u;
+        memcpy(&u, &f, 4);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        return (unsigned short)(u >> 16);
+    };
+    for (int layer = 0; layer < 2; ++layer) {
+        const float* W = w + (layer == 0 ? kW2 : kW3);
+        for (int mt = 0; mt < 4; ++mt)
+            for (int s = 0; s < 16; ++s)
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int i = 0; i < 4; ++i) {
+                        int o = 32 * mt + lane % 32, k = 8 * s + 4 * (lane / 32) + i;
+                        out[(((size_t)layer * 4 * 16 + mt * 16 + s) * 64 + lane) * 4 + i] = bf(W[o * kHidden + k]);
+                    }
+    }
+    return out;
+}
+
+}  // namespace emu
+
+// ----------------------------------------------------------- host entry
+int emu_upload(const float* weights, void** wf, void** wp) {
+    std::vector<unsigned short> packed = emu::pack_bf16(weights);
+    EMU_HIP(hipMalloc(wf, emu::kParams * 4));
+    EMU_HIP(hipMalloc(wp, packed.size() * 2));
+    EMU_HIP(hipMemcpy(*wf, weights, emu::kParams * 4, hipMemcpyHostToDevice));
+    EMU_HIP(hipMemcpy(*wp, packed.data(), packed.size() * 2, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// emulated tables for a launch: nodes -> coefficients -> check -> flags
+int emu_launch_tables(hipStream_t st, const lensed_desc* d_desc, int nsys, int n_nodes, int n_intervals, int n_cands,
+                      const float* params, const void* wf, const void* wp, void* nodes, void* tables, int* flags,
+                      float* errs, float eps) {
+    using namespace emu;
+    int waves = (n_nodes + 31) / 32;
+    hipLaunchKernelGGL(k_emu_nodes, dim3((waves + 3) / 4), dim3(256), 0, st, d_desc, nsys, n_nodes, params,
+                       (const float*)wf, (const bf16x4*)wp, (float4*)nodes);
+    hipLaunchKernelGGL(k_emu_assemble, dim3((n_intervals + 255) / 256), dim3(256), 0, st, d_desc, nsys, n_intervals,
+                       (const float4*)nodes, (float4*)tables);
+    hipLaunchKernelGGL(k_emu_check, dim3((n_cands + 3) / 4), dim3(256), 0, st, d_desc, nsys, n_cands, params,
+                       (const float4*)tables, eps, flags, errs);
+    EMU_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace lsd
+
+using namespace lsd;
+using namespace lsd::emu;
+
+static int write_weights(const char* path, const std::vector<float>& w) {
+    FILE* f = fopen(path, "wb");
+    if (!f) {
+        set_error(std::string("cannot write ") + path);
+        return -1;
+    }
+    const char magic[8] = {'L', 'E', 'M', 'U', '0', '0', '0', '1'};
+    unsigned n = (unsigned)w.size();
+    fwrite(magic, 1, 8, f);
+    fwrite(&n, 4, 1, f);
+    fwrite(w.data(), 4, w.size(), f);
+    fclose(f);
+    return 0;
+}
+
+extern "C" {
+
+int lensed_emulator_read(const char* path, float* weights) {
+    FILE* f = fopen(path, "rb");
+    if (!f) {
+        set_error(std::string("cannot read emulator weights ") + (path ? path : "(null)"));
+        return -1;
+    }
+    char magic[8];
+    unsigned n = 0;
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "LEMU0001", 8) == 0 && fread(&n, 4, 1, f) == 1 &&
+              n == (unsigned)kParams && fread(weights, 4, n, f) == n;
+    fclose(f);
+    if (!ok) set_error(std::string("not an emulator weights file: ") + path);
+    return ok ? 0 : -1;
+}
+
+int lensed_emulator_train(const char* path, long long nsamples, int epochs, int batch, double lr,
+                          unsigned long long seed, double* stats) {
+    if (!path || nsamples <= 0 || (nsamples & (nsamples - 1)) || epochs <= 0 || batch <= 0 || batch > nsamples) {
+        set_error("lensed_emulator_train: invalid arguments (nsamples must be a power of two)");
+        return -1;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st;
+    EMU_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    float *X, *Y, *m, *v;
+    EMU_HIP(hipMalloc(&X, (size_t)nsamples * kIn * 4));
+    EMU_HIP(hipMalloc(&Y, (size_t)nsamples * kOut * 4));
+    EMU_HIP(hipMalloc(&m, kParams * 4));
+    EMU_HIP(hipMalloc(&v, kParams * 4));
+    EMU_HIP(hipMemsetAsync(m, 0, kParams * 4, st));
+    EMU_HIP(hipMemsetAsync(v, 0, kParams * 4, st));
+    hipLaunchKernelGGL(k_make_data, blocks(nsamples), dim3(256), 0, st, nsamples, seed, X, Y);
+    Net net;
+    if (net.alloc(batch)) return -1;
+    // He-style uniform initialisation, seeded
+    std::vector<float> w(kParams, 0.0f);
+    std::mt19937_64 rng(seed);
+    auto init = [&](int off, int fan_in, int count) {
+        std::uniform_real_distribution<float> u(-std::sqrt(3.0f / fan_in), std::sqrt(3.0f / fan_in));
+        for (int i = 0; i < count; ++i) w[off + i] = u(rng);
+    };
+    init(kW1, kIn, kHidden * kIn);
+    init(kW2, kHidden, kHidden * kHidden);
+    init(kW3, kHidden, kHidden * kHidden);
+    init(kW4, kHidden, kOut * kHidden);
+    EMU_HIP(hipMemcpyAsync(net.w, w.data(), kParams * 4, hipMemcpyHostToDevice, st));
+    const long long steps_per_epoch = nsamples / batch;
+    const long long total = steps_per_epoch * epochs;
+    long long step = 0;
+    double last_loss = 0.0;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int e = 0; e < epochs; ++e) {
+        long long stride = (long long)(mix64_host(seed + 1000 + e) | 1ull) & (nsamples - 1);
+        stride |= 1;
+        long long start = (long long)(mix64_host(seed + 2000 + e) & (unsigned long long)(nsamples - 1));
+        EMU_HIP(hipMemsetAsync(net.loss, 0, 8, st));
+        for (long long b = 0; b < steps_per_epoch; ++b, ++step) {
+            hipLaunchKernelGGL(k_gather, blocks(batch), dim3(256), 0, st, batch, nsamples, start + b * batch * stride,
+                               stride, X, Y, net.xb, net.yb);
+            forward(st, net);
+            hipLaunchKernelGGL(k_out_grad, blocks((long long)batch * kOut), dim3(256), 0, st, batch, net.z4,
+                               net.w + kB4, net.yb, net.d4, net.loss);
+            backward(st, net);
+            double frac = (double)step / (double)total;
+            float lrt = (float)(lr * (0.01 + 0.99 * 0.5 * (1 + std::cos(3.14159265358979 * frac))));
+            float c1 = 1.0f - std::pow(b1, (float)(step + 1)), c2 = 1.0f - std::pow(b2, (float)(step + 1));
+            hipLaunchKernelGGL(k_adam, blocks(kParams), dim3(256), 0, st, kParams, net.w, net.g, m, v, lrt, b1, b2,
+                               eps, c1, c2);
+        }
+        double l = 0.0;
+        EMU_HIP(hipMemcpyAsync(&l, net.loss, 8, hipMemcpyDeviceToHost, st));
+        EMU_HIP(hipStreamSynchronize(st));
+        last_loss = l / ((double)steps_per_epoch * batch * kOut);
+    }
+    EMU_HIP(hipMemcpyAsync(w.data(), net.w, kParams * 4, hipMemcpyDeviceToHost, st));
+    EMU_HIP(hipStreamSynchronize(st));
+    net.release();
+    (void)hipFree(X);
+    (void)hipFree(Y);
+    (void)hipFree(m);
+    (void)hipFree(v);
+    (void)hipStreamDestroy(st);
+    if (write_weights(path, w)) return -1;
+    if (stats) {
+        stats[0] = last_loss;
+        stats[1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return 0;
+}
+
+// FP32 forward + backward of the training kernels on n samples: mean
+// squared error loss and its gradient (flat layout) for given weights
+double lensed_emulator_loss_grad(const float* weights, const float* X, const float* Y, int n, float* grad) {
+    hipStream_t st = nullptr;
+    Net net;
+    if (n <= 0 || net.alloc(n)) return NAN;
+    double loss = 0.0;
+    if (hipMemcpy(net.w, weights, kParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(net.xb, X, (size_t)n * kIn * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(net.yb, Y, (size_t)n * kOut * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(net.loss, 0, 8) != hipSuccess) {
+        net.release();
+        return NAN;
+    }
+    forward(st, net);
+    hipLaunchKernelGGL(k_out_grad, blocks((long long)n * kOut), dim3(256), 0, st, n, net.z4, net.w + kB4, net.yb,
+                       net.d4, net.loss);
+    backward(st, net);
+    if (hipMemcpy(grad, net.g, kParams * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&loss, net.loss, 8, hipMemcpyDeviceToHost) != hipSuccess)
+        loss = NAN;
+    net.release();
+    return loss / ((double)n * kOut);
+}
+
+// the network at n inputs X[n][4] with tangent directions dX[n][4]:
+// bf16 = 0 runs the FP32 training forward pass (dY not computed, left 0),
+// bf16 = 1 the BF16-MFMA inference kernel (Y and dY)
+int lensed_emulator_eval(const float* weights, const float* X, const float* dX, int n, int bf16, float* Y, float* dY) {
+    if (n <= 0) return 0;
+    if (!bf16) {
+        Net net;
+        if (net.alloc(n)) return -1;
+        std::vector<float> z(n * kOut), b4(weights + kB4, weights + kB4 + kOut);
+        EMU_HIP(hipMemcpy(net.w, weights, kParams * 4, hipMemcpyHostToDevice));
+        EMU_HIP(hipMemcpy(net.xb, X, (size_t)n * kIn * 4, hipMemcpyHostToDevice));
+        forward(nullptr, net);
+        EMU_HIP(hipMemcpy(z.data(), net.z4, (size_t)n * kOut * 4, hipMemcpyDeviceToHost));
+        net.release();
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < kOut; ++c) {
+                Y[i * kOut + c] = z[i * kOut + c] + b4[c];
+                if (dY) dY[i * kOut + c] = 0.0f;
+            }
+        return 0;
+    }
+    void *wf, *wp;
+    float *dx, *ddx, *dy, *ddy;
+    if (emu_upload(weights, &wf, &wp)) return -1;
+    EMU_HIP(hipMalloc(&dx, (size_t)n * 16));
+    EMU_HIP(hipMalloc(&ddx, (size_t)n * 16));
+    EMU_HIP(hipMalloc(&dy, (size_t)n * 8));
+    EMU_HIP(hipMalloc(&ddy, (size_t)n * 8));
+    EMU_HIP(hipMemcpy(dx, X, (size_t)n * 16, hipMemcpyHostToDevice));
+    EMU_HIP(hipMemcpy(ddx, dX, (size_t)n * 16, hipMemcpyHostToDevice));
+    int waves = (n + 31) / 32;
+    hipLaunchKernelGGL(k_mlp_raw, dim3((waves + 3) / 4), dim3(256), 0, nullptr, n, (const float*)wf,
+                       (const bf16x4*)wp, dx, ddx, dy, ddy);
+    EMU_HIP(hipMemcpy(Y, dy, (size_t)n * 8, hipMemcpyDeviceToHost));
+    if (dY) EMU_HIP(hipMemcpy(dY, ddy, (size_t)n * 8, hipMemcpyDeviceToHost));
+    for (void* p : {wf, wp, (void*)dx, (void*)ddx, (void*)dy, (void*)ddy}) (void)hipFree(p);
+    return 0;
+}
+
+}  // extern "C"

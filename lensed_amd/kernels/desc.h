@@ -32,7 +32,8 @@ struct lensed_desc {
     int item_t, epl_obj, epl_q, epl_t;   // table items; EPL object word offset, q and t param index
     int quad_off, nq, conv_ry, sep_rank;  // nodes (float4 offset, count); conv rows/thread; PSF rank (0 = direct)
     long long sep_off;                   // floats into psf[]: per rank, column [psfh] then row [psfw] factor
-    int npart, table_mode, pad3, pad4;   // chi^2 partials per candidate (8-row x 64-column blocks); EPL table in use
+    int npart, table_mode, item_e, pad4;  // chi^2 partials per candidate (8-row x 64-column blocks); EPL table
+                                          // in use; first emulator node item (2 (NT + 1) nodes per candidate)
     long long psft_off;                  // floats into psf[]: the taps transposed, [psfw][psfh]
 };
 

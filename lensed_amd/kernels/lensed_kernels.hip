@@ -18,6 +18,7 @@
 #ifndef LENSED_FOR_PSFS
 #define LENSED_FOR_PSFS(X)
 #endif
+#include "epl_series.h"
 
 typedef unsigned int u32;
 
@@ -133,39 +134,12 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
 // -------------------------------------------------------------- EPL tables
 // One thread per (candidate, octant, interval): the angular factor Omega and
 // dOmega/dtau of the tenth-order series at both ends of the interval, in
-// FP64 from the candidate's (q, t), turned into cubic Hermite coefficients.
-__device__ inline void lensed_epl_omega(double q, double t, double psi, double& ox, double& oy, double& dx,
-                                        double& dy)
-{
-    const double f = (1 - q) / (1 + q), T = 2 - t;
-    double c = 1.0;
-    ox = oy = dx = dy = 0.0;
-    for (int k = 0; k <= 10; ++k) {
-        if (k > 0) c *= -f * (2 * k - T) / (2 * k + T);
-        double m = 2 * k + 1, sn, cs;
-        sincos(m * psi, &sn, &cs);
-        ox += c * cs;
-        oy += c * sn;
-        dx -= c * m * sn;
-        dy += c * m * cs;
-    }
-}
-
-__device__ inline void lensed_epl_node(double q, double t, int half, double tau, double& ox, double& oy,
-                                       double& dx, double& dy)
-{
-    double a = atan(tau);
-    double psi = half ? 1.57079632679489661923 - a : a;
-    double gx, gy;
-    lensed_epl_omega(q, t, psi, ox, oy, gx, gy);
-    double dpsi = (half ? -1.0 : 1.0) / (1 + tau * tau) / LENSED_EPL_NT;  // d psi / d f
-    dx = gx * dpsi;
-    dy = gy * dpsi;
-}
-
+// FP64 from the candidate's (q, t), turned into cubic Hermite coefficients
+// (epl_series.h).  With `only` set, just the flagged candidates (the
+// emulator's fallback).
 extern "C" __global__ __launch_bounds__(256) void lensed_epl_tables(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
-    const float* __restrict__ params, float4* __restrict__ tables)
+    const float* __restrict__ params, float4* __restrict__ tables, const int* __restrict__ only)
 {
     int item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= nitems) return;
@@ -175,16 +149,15 @@ extern "C" __global__ __launch_bounds__(256) void lensed_epl_tables(
     int cand = local / (2 * LENSED_EPL_NT);
     int rem = local - cand * 2 * LENSED_EPL_NT;
     int half = rem / LENSED_EPL_NT, i = rem - half * LENSED_EPL_NT;
+    if (only && !only[d.table_off / LENSED_EPL_TABLE_F4 + cand]) return;
     const float* p = params + d.param_off + (long long)cand * d.npar;
     double q = p[d.epl_q], t = p[d.epl_t];
     double x0, y0, dx0, dy0, x1, y1, dx1, dy1;
     lensed_epl_node(q, t, half, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
     lensed_epl_node(q, t, half, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
     float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4 + 2 * rem;
-    out[0] = make_float4((float)x0, (float)dx0, (float)(3 * (x1 - x0) - 2 * dx0 - dx1),
-                         (float)(2 * (x0 - x1) + dx0 + dx1));
-    out[1] = make_float4((float)y0, (float)dy0, (float)(3 * (y1 - y0) - 2 * dy0 - dy1),
-                         (float)(2 * (y0 - y1) + dy0 + dy1));
+    out[0] = lensed_hermite(x0, dx0, x1, dx1);
+    out[1] = lensed_hermite(y0, dy0, y1, dy1);
 }
 
 // --------------------------------------------------------------- conv+chi2

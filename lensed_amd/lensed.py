@@ -157,13 +157,14 @@ class SystemSpec:
 class Context:
     """The likelihood context for one or more systems (struct lensed)."""
 
-    def __init__(self, systems, flags=0, max_batch=1, device=0):
+    def __init__(self, systems, flags=0, max_batch=1, device=0, emulator=None, emulator_tol=0.0):
         if isinstance(systems, SystemSpec):
             systems = [systems]
         self.systems = list(systems)
         lib = L.load()
         structs = (L.System * len(self.systems))(*[s._struct() for s in self.systems])
-        opts = L.Options(device=device, flags=flags, max_batch=max_batch, emulator=None)
+        opts = L.Options(device=device, flags=flags, max_batch=max_batch,
+                         emulator=None if emulator is None else str(emulator).encode(), emulator_tol=emulator_tol)
         self.ptr = lib.lensed_create(structs, len(self.systems), byref(opts))
         if not self.ptr:
             raise LensedError(L.last_error())
@@ -240,6 +241,13 @@ class Context:
                                       planes["image"].ctypes.data_as(fp), planes["chi2"].ctypes.data_as(fp)),
                "render")
         return planes
+
+    def emulator_stats(self):
+        """(fallbacks, candidates checked, largest held-out error) of the last emulated launch."""
+        from ctypes import c_long
+        nf, nc, mx = c_long(), c_long(), c_double()
+        _check(L.load().lensed_emulator_stats(self.ptr, byref(nf), byref(nc), byref(mx)), "emulator_stats")
+        return nf.value, nc.value, mx.value
 
     def profile(self):
         """Accumulated stage times (ms) with the reference's stage names."""
