@@ -227,6 +227,57 @@ int lensed_emulator_eval(const float* weights, const float* X, const float* dX, 
  * fell back to the exact series), candidates checked, largest held-out error */
 int lensed_emulator_stats(const lensed* l, long* fallbacks, long* checked, double* max_err);
 
+/* ------------------------------------------------------ nested sampling */
+
+/* src/lensed.c:1270-1275 -- MultiNest's C entry point, with the arguments
+ * Lensed passes.  Provided here by a single-ellipsoid nested sampler that
+ * calls LogLike once per candidate and Dumper every updInt iterations; IS,
+ * mmodal, nClsPar, maxModes, Ztol, fb, initMPI and logZero are accepted and
+ * ignored (no importance weighting, mode separation or MPI).  efr is the
+ * target acceptance when ceff is set, else the shrinking factor.  With a
+ * root and resume = 1 the run continues from <root>resume.dat; outfile = 1
+ * writes <root>.txt (weight, -2 logL, parameters per posterior sample). */
+void run(int IS, int mmodal, int ceff, int nlive, double tol, double efr, int ndims, int nPar, int nClsPar,
+         int maxModes, int updInt, double Ztol, char root[], int seed, int* pWrap, int fb, int resume,
+         int outfile, int initMPI, double logZero, int maxiter,
+         void (*LogLike)(double*, int*, int*, double*, void*),
+         void (*Dumper)(int*, int*, int*, double**, double**, double**, double*, double*, double*, double*,
+                        void*),
+         void* context);
+
+/* The same sampler proposing `batch` candidates at a time; no counterpart in
+ * the reference (MultiNest proposes one point per likelihood call). */
+typedef struct {
+    int nlive;       /* live points (300) */
+    double tol;      /* stop when the live points can raise ln Z by < tol (0.1) */
+    double efr;      /* target acceptance if ceff, else shrinking factor (0.8) */
+    int ceff;        /* constant efficiency mode */
+    int batch;       /* candidates proposed per likelihood call */
+    int seed;        /* random seed */
+    int updint;      /* iterations between dumper calls (0 = only at the end) */
+    int maxiter;     /* iteration limit (0 = none) */
+    int resume;      /* continue from <root>resume.dat */
+    int outfile;     /* write <root>.txt */
+    const int* wrap; /* npar periodic flags (sampler order), or NULL */
+    const char* root;/* output prefix, or NULL */
+} lensed_ns_options;
+
+typedef struct {
+    double logz, logz_err, info, max_loglike;
+    long nevals, niter;
+} lensed_ns_result;
+
+/* cubes[n][ndim] -> phys[n][npar] (sampler order) and lnew[n] */
+typedef void (*lensed_batch_fn)(const double* cubes, int n, double* phys, double* lnew, void* ctx);
+
+/* a generic likelihood; mean, sigma and ml (npar each, sampler order) may be NULL */
+int lensed_nested_sample(int ndim, int npar, lensed_batch_fn fn, void* ctx, const lensed_ns_options* opts,
+                         double* mean, double* sigma, double* ml, lensed_ns_result* result);
+/* system s of a context through lensed_loglike_batch(); for system 0 the
+ * results are also recorded as dumper() records them */
+int lensed_nested_run(lensed* l, size_t s, const lensed_ns_options* opts, double* mean, double* sigma,
+                      double* ml, lensed_ns_result* result);
+
 /* results recorded by dumper(): mean, sigma, ML, MAP per parameter in object
  * order (each may be NULL) and the log-evidence summary */
 int lensed_results(const lensed* l, double* mean, double* sigma, double* ml, double* map,

@@ -36,6 +36,23 @@ class Options(Structure):
                 ("emulator_tol", c_double)]
 
 
+class NsOptions(Structure):
+    _fields_ = [("nlive", c_int), ("tol", c_double), ("efr", c_double), ("ceff", c_int), ("batch", c_int),
+                ("seed", c_int), ("updint", c_int), ("maxiter", c_int), ("resume", c_int), ("outfile", c_int),
+                ("wrap", POINTER(c_int)), ("root", c_char_p)]
+
+
+class NsResult(Structure):
+    _fields_ = [("logz", c_double), ("logz_err", c_double), ("info", c_double), ("max_loglike", c_double),
+                ("nevals", c_long), ("niter", c_long)]
+
+
+BATCH_FN = ctypes.CFUNCTYPE(None, POINTER(c_double), c_int, POINTER(c_double), POINTER(c_double), c_void_p)
+LOGLIKE_FN = ctypes.CFUNCTYPE(None, POINTER(c_double), POINTER(c_int), POINTER(c_int), POINTER(c_double), c_void_p)
+DUMPER_FN = ctypes.CFUNCTYPE(None, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(POINTER(c_double)),
+                             POINTER(POINTER(c_double)), POINTER(POINTER(c_double)), POINTER(c_double),
+                             POINTER(c_double), POINTER(c_double), POINTER(c_double), c_void_p)
+
 # every entry point declared in include/lensed.h: name -> (restype, argtypes)
 SIGNATURES = {
     "object_program": (None, [c_char_p, POINTER(c_size_t), POINTER(POINTER(c_char_p))]),
@@ -83,6 +100,13 @@ SIGNATURES = {
     "lensed_emulator_eval": (c_int, [POINTER(c_float), POINTER(c_float), POINTER(c_float), c_int, c_int,
                                      POINTER(c_float), POINTER(c_float)]),
     "lensed_emulator_stats": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
+    "run": (None, [c_int, c_int, c_int, c_int, c_double, c_double, c_int, c_int, c_int, c_int, c_int, c_double,
+                   c_char_p, c_int, POINTER(c_int), c_int, c_int, c_int, c_int, c_double, c_int, LOGLIKE_FN,
+                   DUMPER_FN, c_void_p]),
+    "lensed_nested_sample": (c_int, [c_int, c_int, BATCH_FN, c_void_p, POINTER(NsOptions), POINTER(c_double),
+                                     POINTER(c_double), POINTER(c_double), POINTER(NsResult)]),
+    "lensed_nested_run": (c_int, [c_void_p, c_size_t, POINTER(NsOptions), POINTER(c_double), POINTER(c_double),
+                                  POINTER(c_double), POINTER(NsResult)]),
     "lensed_results": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double),
                                POINTER(c_double), POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
 }

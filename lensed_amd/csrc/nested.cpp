@@ -1,0 +1,540 @@
+// Nested sampling driver (the caller of the likelihood path).
+//
+// Lensed hands its likelihood to MultiNest's run() (src/lensed.c:1236-1288),
+// which calls loglike() once per candidate and dumper() every updInt
+// iterations (src/nested.c:17,133).  MultiNest is an external Fortran library
+// that is not part of this build; this file provides a nested sampler with
+// the same entry point and callbacks, and a batched variant that proposes K
+// points at a time so one launch evaluates them all.
+//
+// Algorithm (Skilling 2004; single bounding ellipsoid as in Mukherjee,
+// Parkinson & Liddle 2006): nlive points uniform in the unit cube; at each
+// iteration the lowest-likelihood point dies with prior-volume shrinkage
+// X_i = exp(-i / nlive) and is replaced by a draw from the prior restricted
+// to L > L_min.  Draws come from the ellipsoid bounding the live points
+// (covariance-shaped, scaled to contain them all, enlarged), rejected where
+// L <= L_min or outside the cube (wrapped for periodic parameters).  Proposals
+// are drawn in batches; a batch is consumed in order against the rising
+// threshold, which keeps every accepted point a uniform draw from the
+// constrained prior as long as the ellipsoid still bounds it.  The evidence
+// accumulates Z += L_i (X_{i-1} - X_i); the run stops when the live points
+// can raise ln Z by at most tol (MultiNest's tolerance), and the live points
+// are then added with the remaining volume.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/lensed.h"
+
+namespace lsd {
+void set_error(const std::string& msg);
+}
+
+namespace {
+
+struct Point {
+    std::vector<double> u;     // unit cube, ndim
+    std::vector<double> phys;  // physical parameters, npar (sampler order)
+    double logl;
+};
+
+double logaddexp(double a, double b) {
+    if (a == -INFINITY) return b;
+    if (b == -INFINITY) return a;
+    double m = std::max(a, b);
+    return m + std::log(std::exp(a - m) + std::exp(b - m));
+}
+
+// Cholesky of a symmetric positive-definite n x n matrix (row-major, lower)
+bool cholesky(std::vector<double>& a, int n) {
+    for (int j = 0; j < n; ++j) {
+        double s = a[j * n + j];
+        for (int k = 0; k < j; ++k) s -= a[j * n + k] * a[j * n + k];
+        if (s <= 0) return false;
+        a[j * n + j] = std::sqrt(s);
+        for (int i = j + 1; i < n; ++i) {
+            double t = a[i * n + j];
+            for (int k = 0; k < j; ++k) t -= a[i * n + k] * a[j * n + k];
+            a[i * n + j] = t / a[j * n + j];
+        }
+        for (int k = j + 1; k < n; ++k) a[j * n + k] = 0;
+    }
+    return true;
+}
+
+struct Ellipsoid {
+    int n = 0;
+    std::vector<double> mean, chol;  // x = mean + scale * chol * z, |z| <= 1
+    std::vector<double> shift;       // periodic coordinates are fitted as frac(u + shift)
+    double scale = 1;
+
+    double coord(const Point& p, int i) const {
+        double v = p.u[i] + shift[i];
+        return shift[i] != 0 ? v - std::floor(v) : v;
+    }
+
+    // bound the points, then enlarge linearly by `enlarge`; a periodic
+    // coordinate is first rotated so that the circular mean of the points
+    // sits at 0.5, which keeps a mode across the 0/1 seam in one piece
+    bool fit(const std::vector<Point>& pts, double enlarge, const std::vector<int>& wrap) {
+        n = (int)pts[0].u.size();
+        int m = (int)pts.size();
+        shift.assign(n, 0.0);
+        for (int i = 0; i < n && i < (int)wrap.size(); ++i) {
+            if (!wrap[i]) continue;
+            double cs = 0, sn = 0;
+            for (const auto& p : pts) {
+                cs += std::cos(2 * M_PI * p.u[i]);
+                sn += std::sin(2 * M_PI * p.u[i]);
+            }
+            double c = std::atan2(sn, cs) / (2 * M_PI);
+            shift[i] = 0.5 - (c - std::floor(c));
+            if (shift[i] == 0) shift[i] = 1;  // still marks the coordinate periodic
+        }
+        mean.assign(n, 0.0);
+        for (const auto& p : pts)
+            for (int i = 0; i < n; ++i) mean[i] += coord(p, i) / m;
+        std::vector<double> cov((size_t)n * n, 0.0);
+        for (const auto& p : pts)
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j <= i; ++j)
+                    cov[i * n + j] += (coord(p, i) - mean[i]) * (coord(p, j) - mean[j]) / (m - 1);
+        for (int i = 0; i < n; ++i) {
+            cov[i * n + i] += 1e-12;
+            for (int j = 0; j < i; ++j) cov[j * n + i] = cov[i * n + j];
+        }
+        chol = cov;
+        if (!cholesky(chol, n)) {  // degenerate live set: the sphere around the whole cube
+            std::fill(shift.begin(), shift.end(), 0.0);
+            mean.assign(n, 0.5);
+            chol.assign((size_t)n * n, 0.0);
+            for (int i = 0; i < n; ++i) chol[i * n + i] = 1;
+            scale = 0.5 * std::sqrt((double)n);
+            return false;
+        }
+        // largest Mahalanobis radius of the points: solve L y = x - mean
+        double rmax = 0;
+        std::vector<double> y(n);
+        for (const auto& p : pts) {
+            double r2 = 0;
+            for (int i = 0; i < n; ++i) {
+                double t = coord(p, i) - mean[i];
+                for (int k = 0; k < i; ++k) t -= chol[i * n + k] * y[k];
+                y[i] = t / chol[i * n + i];
+                r2 += y[i] * y[i];
+            }
+            rmax = std::max(rmax, r2);
+        }
+        scale = std::sqrt(rmax) * enlarge;
+        return true;
+    }
+
+    void sample(std::mt19937_64& rng, double* out) const {
+        std::normal_distribution<double> g(0, 1);
+        std::uniform_real_distribution<double> u(0, 1);
+        std::vector<double> z(n);
+        double r2 = 0;
+        for (int i = 0; i < n; ++i) {
+            z[i] = g(rng);
+            r2 += z[i] * z[i];
+        }
+        double rad = std::pow(u(rng), 1.0 / n) / std::sqrt(r2);
+        for (int i = 0; i < n; ++i) {
+            double s = 0;
+            for (int k = 0; k <= i; ++k) s += chol[i * n + k] * z[k];
+            out[i] = mean[i] + scale * rad * s - shift[i];
+        }
+    }
+};
+
+struct Options {
+    int nlive = 300, batch = 1, seed = 0, updint = 1000, maxiter = 0, ceff = 0, resume = 0, outfile = 0;
+    double tol = 0.1, efr = 0.8;
+    std::vector<int> wrap;
+    std::string root;
+};
+
+typedef void (*BatchFn)(const double* cubes, int n, double* phys, double* lnew, void* ctx);
+
+struct Result {
+    double logz = -INFINITY, logz_err = 0, info = 0, max_logl = -INFINITY;
+    long nevals = 0, niter = 0;
+    std::vector<double> mean, sigma, ml;
+    std::vector<Point> dead;
+    std::vector<double> dead_logw;  // log prior-volume widths
+};
+
+bool inside(const double* u, int n, const std::vector<int>& wrap, double* out) {
+    for (int i = 0; i < n; ++i) {
+        double v = u[i];
+        if (i < (int)wrap.size() && wrap[i]) v -= std::floor(v);
+        if (v < 0 || v > 1) return false;
+        out[i] = v;
+    }
+    return true;
+}
+
+void write_checkpoint(const std::string& path, const std::vector<Point>& live, const Result& r, int ndim, int npar) {
+    FILE* f = fopen(path.c_str(), "w");
+    if (!f) return;
+    fprintf(f, "lensed-nested 1 %d %d %zu %zu %ld %ld\n", ndim, npar, live.size(), r.dead.size(), r.niter, r.nevals);
+    auto put = [&](const Point& p, double lw) {
+        fprintf(f, "%.17g %.17g", p.logl, lw);
+        for (double v : p.u) fprintf(f, " %.17g", v);
+        for (double v : p.phys) fprintf(f, " %.17g", v);
+        fprintf(f, "\n");
+    };
+    for (const auto& p : live) put(p, 0);
+    for (size_t i = 0; i < r.dead.size(); ++i) put(r.dead[i], r.dead_logw[i]);
+    fprintf(f, "%.17g %.17g %.17g\n", r.logz, r.info, r.max_logl);
+    fclose(f);
+}
+
+bool read_checkpoint(const std::string& path, std::vector<Point>& live, Result& r, int ndim, int npar) {
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return false;
+    int nd, np;
+    size_t nl, ndd;
+    bool ok = fscanf(f, "lensed-nested 1 %d %d %zu %zu %ld %ld", &nd, &np, &nl, &ndd, &r.niter, &r.nevals) == 6 &&
+              nd == ndim && np == npar;
+    auto get = [&](Point& p, double& lw) {
+        p.u.resize(ndim);
+        p.phys.resize(npar);
+        bool good = fscanf(f, "%lf %lf", &p.logl, &lw) == 2;
+        for (int i = 0; i < ndim && good; ++i) good = fscanf(f, "%lf", &p.u[i]) == 1;
+        for (int i = 0; i < npar && good; ++i) good = fscanf(f, "%lf", &p.phys[i]) == 1;
+        return good;
+    };
+    live.assign(ok ? nl : 0, Point());
+    double lw;
+    for (size_t i = 0; ok && i < nl; ++i) ok = get(live[i], lw);
+    r.dead.assign(ok ? ndd : 0, Point());
+    r.dead_logw.assign(ok ? ndd : 0, 0.0);
+    for (size_t i = 0; ok && i < ndd; ++i) ok = get(r.dead[i], r.dead_logw[i]);
+    ok = ok && fscanf(f, "%lf %lf %lf", &r.logz, &r.info, &r.max_logl) == 3;
+    fclose(f);
+    if (!ok) {  // unusable checkpoint: start afresh
+        r = Result();
+        live.clear();
+    }
+    return ok;
+}
+
+// posterior weights of dead + final live points -> mean, sigma, ML
+void summarise(Result& r, const std::vector<Point>& live, double logx, int npar) {
+    std::vector<const Point*> pts;
+    std::vector<double> lw;
+    for (size_t i = 0; i < r.dead.size(); ++i) {
+        pts.push_back(&r.dead[i]);
+        lw.push_back(r.dead_logw[i] + r.dead[i].logl - r.logz);
+    }
+    double lwl = logx - std::log((double)live.size());
+    for (const auto& p : live) {
+        pts.push_back(&p);
+        lw.push_back(lwl + p.logl - r.logz);
+    }
+    r.mean.assign(npar, 0.0);
+    r.sigma.assign(npar, 0.0);
+    r.ml.assign(npar, 0.0);
+    double wsum = 0, best = -INFINITY;
+    for (size_t i = 0; i < pts.size(); ++i) {
+        double w = std::exp(lw[i]);
+        wsum += w;
+        for (int k = 0; k < npar; ++k) r.mean[k] += w * pts[i]->phys[k];
+        if (pts[i]->logl > best) {
+            best = pts[i]->logl;
+            r.ml = pts[i]->phys;
+        }
+    }
+    for (int k = 0; k < npar; ++k) r.mean[k] /= wsum;
+    for (size_t i = 0; i < pts.size(); ++i) {
+        double w = std::exp(lw[i]) / wsum;
+        for (int k = 0; k < npar; ++k) r.sigma[k] += w * (pts[i]->phys[k] - r.mean[k]) * (pts[i]->phys[k] - r.mean[k]);
+    }
+    for (int k = 0; k < npar; ++k) r.sigma[k] = std::sqrt(r.sigma[k]);
+    r.max_logl = best;
+}
+
+typedef void (*DumperFn)(int*, int*, int*, double**, double**, double**, double*, double*, double*, double*, void*);
+
+void call_dumper(DumperFn dumper, void* ctx, Result& r, const std::vector<Point>& live, double logx, int npar) {
+    if (!dumper) return;
+    summarise(r, live, logx, npar);
+    int ns = (int)r.dead.size(), nl = (int)live.size(), np = npar;
+    // MultiNest's arrays, Fortran (column-major) layout
+    std::vector<double> physlive((size_t)nl * (npar + 1)), post((size_t)std::max(ns, 1) * (npar + 2)),
+        cons((size_t)4 * npar);
+    for (int i = 0; i < nl; ++i) {
+        for (int k = 0; k < npar; ++k) physlive[(size_t)k * nl + i] = live[i].phys[k];
+        physlive[(size_t)npar * nl + i] = live[i].logl;
+    }
+    for (int i = 0; i < ns; ++i) {
+        for (int k = 0; k < npar; ++k) post[(size_t)k * ns + i] = r.dead[i].phys[k];
+        post[(size_t)npar * ns + i] = r.dead[i].logl;
+        post[(size_t)(npar + 1) * ns + i] = std::exp(r.dead_logw[i] + r.dead[i].logl - r.logz);
+    }
+    for (int k = 0; k < npar; ++k) {
+        cons[k] = r.mean[k];
+        cons[npar + k] = r.sigma[k];
+        cons[2 * npar + k] = r.ml[k];
+        cons[3 * npar + k] = r.ml[k];  // MAP = ML for the uniform cube prior
+    }
+    double* pl = physlive.data();
+    double* po = post.data();
+    double* co = cons.data();
+    double maxl = r.max_logl, logz = r.logz, ins = r.logz, err = r.logz_err;
+    dumper(&ns, &nl, &np, &pl, &po, &co, &maxl, &logz, &ins, &err, ctx);
+}
+
+int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn dumper, void* dctx, Result& r) {
+    if (ndim <= 0 || o.nlive <= ndim) {
+        lsd::set_error("nested sampling needs 0 < ndim < nlive");
+        return -1;
+    }
+    std::mt19937_64 rng((unsigned long long)o.seed * 6364136223846793005ull + 1442695040888963407ull);
+    std::uniform_real_distribution<double> uni(0, 1);
+    std::vector<Point> live;
+    std::string ckpt = o.root.empty() ? "" : o.root + "resume.dat";
+    bool resumed = o.resume && !ckpt.empty() && read_checkpoint(ckpt, live, r, ndim, npar);
+    if (resumed) rng.seed(rng() ^ (unsigned long long)r.niter);  // a fresh stream after the checkpoint
+    if (!resumed) {
+        live.assign(o.nlive, Point());
+        std::vector<double> cubes((size_t)o.nlive * ndim), phys((size_t)o.nlive * npar), ll(o.nlive);
+        for (auto& c : cubes) c = uni(rng);
+        fn(cubes.data(), o.nlive, phys.data(), ll.data(), ctx);
+        r.nevals += o.nlive;
+        for (int i = 0; i < o.nlive; ++i) {
+            live[i].u.assign(cubes.begin() + (size_t)i * ndim, cubes.begin() + (size_t)(i + 1) * ndim);
+            live[i].phys.assign(phys.begin() + (size_t)i * npar, phys.begin() + (size_t)(i + 1) * npar);
+            live[i].logl = ll[i];
+        }
+    }
+    const int n = o.nlive;
+    const double shrink = -1.0 / n;  // log X_i - log X_{i-1}
+    double logx = shrink * r.niter;
+    // enlargement: without ceff, efr is MultiNest's shrinking factor and the
+    // bounding volume grows by 1/efr (at least 10% in every direction); with
+    // ceff, efr is the target acceptance and the enlargement adapts to it,
+    // never below the tight bound of the live points
+    double enlarge = o.ceff ? 1.2 : std::max(1.1, std::pow(1.0 / o.efr, 1.0 / ndim));
+    Ellipsoid ell;
+    std::vector<double> cubes, phys, ll;
+    size_t next = 0, have = 0;
+    int since_fit = n;
+    long accepted = 0, proposed = 0;
+    while (true) {
+        int worst = 0;
+        for (int i = 1; i < n; ++i)
+            if (live[i].logl < live[worst].logl) worst = i;
+        double lmin = live[worst].logl;
+        // termination: the live points can raise Z by at most a fraction tol
+        double lmax = -INFINITY;
+        for (const auto& p : live) lmax = std::max(lmax, p.logl);
+        // (MultiNest's criterion: ln(Z + dZ_live) - ln Z < tol)
+        if (r.niter > n && lmax + logx < r.logz + std::log(std::expm1(o.tol))) break;
+        if (o.maxiter > 0 && r.niter >= o.maxiter) break;
+        // the dead point: weight width X_{i-1} - X_i
+        double logw = logx + std::log1p(-std::exp(shrink));
+        double logz_new = logaddexp(r.logz, logw + lmin);
+        r.info = std::exp(logw + lmin - logz_new) * lmin +
+                 (r.logz == -INFINITY ? 0 : std::exp(r.logz - logz_new) * (r.info + r.logz)) - logz_new;
+        r.logz = logz_new;
+        r.dead.push_back(live[worst]);
+        r.dead_logw.push_back(logw);
+        logx += shrink;
+        r.niter += 1;
+        // replacement from the constrained prior
+        bool done = false;
+        while (!done) {
+            if (next >= have) {
+                if (since_fit >= std::max(1, n / 10) || ell.n == 0) {
+                    ell.fit(live, enlarge, o.wrap);
+                    since_fit = 0;
+                }
+                int b = std::max(1, o.batch);
+                cubes.resize((size_t)b * ndim);
+                phys.resize((size_t)b * npar);
+                ll.resize(b);
+                for (int k = 0; k < b; ++k) {
+                    std::vector<double> x(ndim);
+                    for (int tries = 0;; ++tries) {
+                        ell.sample(rng, x.data());
+                        if (inside(x.data(), ndim, o.wrap, &cubes[(size_t)k * ndim])) break;
+                        if (tries > 100000) {  // ellipsoid far outside the cube: fall back to the prior
+                            for (int i = 0; i < ndim; ++i) cubes[(size_t)k * ndim + i] = uni(rng);
+                            break;
+                        }
+                    }
+                }
+                fn(cubes.data(), b, phys.data(), ll.data(), ctx);
+                r.nevals += b;
+                next = 0;
+                have = b;
+            }
+            size_t k = next++;
+            proposed += 1;
+            if (ll[k] > lmin) {
+                live[worst].u.assign(cubes.begin() + k * ndim, cubes.begin() + (k + 1) * ndim);
+                live[worst].phys.assign(phys.begin() + k * npar, phys.begin() + (k + 1) * npar);
+                live[worst].logl = ll[k];
+                accepted += 1;
+                since_fit += 1;
+                done = true;
+            }
+        }
+        // constant efficiency mode: steer the enlargement towards acceptance efr
+        if (o.ceff && proposed >= 100) {
+            double acc = (double)accepted / proposed;
+            enlarge *= acc < o.efr ? 0.98 : 1.02;
+            enlarge = std::min(4.0, std::max(1.0, enlarge));
+            accepted = proposed = 0;
+        }
+        if (o.updint > 0 && r.niter % o.updint == 0) {
+            call_dumper(dumper, dctx, r, live, logx, npar);
+            if (!ckpt.empty()) write_checkpoint(ckpt, live, r, ndim, npar);
+        }
+    }
+    // remaining live points share the last volume
+    for (const auto& p : live) {
+        double lw = logx - std::log((double)n);
+        double lz = logaddexp(r.logz, lw + p.logl);
+        r.info = std::exp(lw + p.logl - lz) * p.logl + std::exp(r.logz - lz) * (r.info + r.logz) - lz;
+        r.logz = lz;
+    }
+    r.logz_err = std::sqrt(std::max(r.info, 0.0) / n);
+    summarise(r, live, logx, npar);
+    if (!ckpt.empty()) write_checkpoint(ckpt, live, r, ndim, npar);
+    if (o.outfile && !o.root.empty()) {
+        FILE* f = fopen((o.root + ".txt").c_str(), "w");
+        if (f) {  // MultiNest's <root>.txt: weight, -2 logL, parameters
+            for (size_t i = 0; i < r.dead.size(); ++i) {
+                fprintf(f, "%.10e %.10e", std::exp(r.dead_logw[i] + r.dead[i].logl - r.logz), -2 * r.dead[i].logl);
+                for (double v : r.dead[i].phys) fprintf(f, " %.10e", v);
+                fprintf(f, "\n");
+            }
+            fclose(f);
+        }
+    }
+    call_dumper(dumper, dctx, r, live, logx, npar);
+    return 0;
+}
+
+// the MultiNest callback evaluated one candidate at a time
+struct SerialCtx {
+    void (*loglike)(double*, int*, int*, double*, void*);
+    void* ctx;
+    int ndim, npar;
+};
+
+void serial_batch(const double* cubes, int n, double* phys, double* lnew, void* c) {
+    SerialCtx* s = (SerialCtx*)c;
+    std::vector<double> cube(s->npar);
+    for (int i = 0; i < n; ++i) {
+        std::fill(cube.begin(), cube.end(), 0.0);
+        std::copy(cubes + (size_t)i * s->ndim, cubes + (size_t)(i + 1) * s->ndim, cube.begin());
+        s->loglike(cube.data(), &s->ndim, &s->npar, &lnew[i], s->ctx);
+        std::copy(cube.begin(), cube.end(), phys + (size_t)i * s->npar);
+    }
+}
+
+struct BatchCtx {
+    lensed* l;
+    size_t s;
+    int ndim;
+};
+
+void lensed_batch(const double* cubes, int n, double* phys, double* lnew, void* c) {
+    BatchCtx* b = (BatchCtx*)c;
+    if (lensed_loglike_batch(b->l, b->s, cubes, (size_t)n, phys, lnew))
+        for (int i = 0; i < n; ++i) lnew[i] = -DBL_MAX;
+}
+
+Options to_options(const lensed_ns_options* in, int npar) {
+    Options o;
+    if (!in) return o;
+    o.nlive = in->nlive > 0 ? in->nlive : 300;
+    o.tol = in->tol > 0 ? in->tol : 0.1;
+    o.efr = in->efr > 0 ? in->efr : 0.8;
+    o.ceff = in->ceff;
+    o.batch = in->batch > 0 ? in->batch : 1;
+    o.seed = in->seed;
+    o.updint = in->updint;
+    o.maxiter = in->maxiter;
+    o.resume = in->resume;
+    o.outfile = in->outfile;
+    if (in->wrap) o.wrap.assign(in->wrap, in->wrap + npar);
+    if (in->root) o.root = in->root;
+    return o;
+}
+
+void fill_result(const Result& r, lensed_ns_result* out, double* mean, double* sigma, double* ml, int npar) {
+    if (out) {
+        out->logz = r.logz;
+        out->logz_err = r.logz_err;
+        out->info = r.info;
+        out->max_loglike = r.max_logl;
+        out->nevals = r.nevals;
+        out->niter = r.niter;
+    }
+    for (int k = 0; k < npar; ++k) {
+        if (mean) mean[k] = r.mean[k];
+        if (sigma) sigma[k] = r.sigma[k];
+        if (ml) ml[k] = r.ml[k];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lensed_nested_sample(int ndim, int npar, lensed_batch_fn fn, void* ctx, const lensed_ns_options* opts,
+                         double* mean, double* sigma, double* ml, lensed_ns_result* result) {
+    Options o = to_options(opts, npar);
+    Result r;
+    if (sample(ndim, npar, (BatchFn)fn, ctx, o, nullptr, nullptr, r)) return -1;
+    fill_result(r, result, mean, sigma, ml, npar);
+    return 0;
+}
+
+int lensed_nested_run(lensed* l, size_t s, const lensed_ns_options* opts, double* mean, double* sigma, double* ml,
+                      lensed_ns_result* result) {
+    int ndim = (int)lensed_ndims(l, s), npar = (int)lensed_npars(l, s);
+    BatchCtx b{l, s, ndim};
+    Options o = to_options(opts, npar);
+    Result r;
+    if (sample(ndim, npar, lensed_batch, &b, o, s == 0 ? dumper : nullptr, l, r)) return -1;
+    fill_result(r, result, mean, sigma, ml, npar);
+    return 0;
+}
+
+void run(int IS, int mmodal, int ceff, int nlive, double tol, double efr, int ndims, int nPar, int nClsPar,
+         int maxModes, int updInt, double Ztol, char root[], int seed, int* pWrap, int fb, int resume, int outfile,
+         int initMPI, double logZero, int maxiter, void (*LogLike)(double*, int*, int*, double*, void*),
+         void (*Dumper)(int*, int*, int*, double**, double**, double**, double*, double*, double*, double*, void*),
+         void* context) {
+    (void)IS; (void)mmodal; (void)nClsPar; (void)maxModes; (void)Ztol; (void)fb; (void)initMPI; (void)logZero;
+    Options o;
+    o.nlive = nlive;
+    o.tol = tol;
+    o.efr = efr;
+    o.ceff = ceff;
+    o.updint = updInt;
+    o.seed = seed < 0 ? (int)std::random_device()() : seed;
+    o.maxiter = maxiter;
+    o.resume = resume;
+    o.outfile = outfile;
+    o.batch = 1;
+    if (pWrap) o.wrap.assign(pWrap, pWrap + nPar);
+    if (root) o.root = root;
+    SerialCtx sc{LogLike, context, ndims, nPar};
+    Result r;
+    if (sample(ndims, nPar, serial_batch, &sc, o, Dumper, context, r))
+        fprintf(stderr, "lensed: nested sampling failed\n");
+}
+
+}  // extern "C"

@@ -20,6 +20,7 @@ OBJECTS = sorted(O.PARAMS)
 def header_functions():
     text = open(os.path.join(ROOT, "include", "lensed.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"\btypedef\b[^;]*;", "", text)
     return sorted(set(re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text)))
 
 

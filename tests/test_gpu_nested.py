@@ -1,0 +1,87 @@
+"""Nested sampling end to end on the GPU likelihood (C1: SIE + Sersic, 100 x 100).
+
+The plain run is Lensed's own call (src/lensed.c:1270-1275): run() with the
+library's loglike and dumper, one candidate per likelihood call.  The
+batched run proposes 1024 candidates per launch (lensed_nested_run).  The
+two are independent samplings of the same posterior, so they must agree
+within their sampling noise: log-evidence within four combined standard
+errors, posterior means within 0.3 sigma, and the mock's truth inside the
+posterior.
+"""
+import ctypes
+import math
+import time
+from ctypes import byref, c_double
+
+import numpy as np
+import pytest
+
+from lensed_amd import _lib as L
+from lensed_amd import nested
+from lensed_amd import workloads as W
+from lensed_amd.lensed import Context, LensedObject, SystemSpec
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_render_cb(pids):
+    def render(objects, params, shape, psf):
+        pri = {k: repr(float(v)) for k, v in zip(pids, params)}
+        spec = SystemSpec(objects, pri, np.zeros(shape, np.float32), np.ones(shape, np.float32), psf=psf)
+        with Context(spec, max_batch=1) as c:
+            return c.render(params)["image"].astype(np.float64)
+    return render
+
+
+@pytest.fixture(scope="module")
+def c1_ctx():
+    wl = W.c1()
+    objs = [LensedObject(oid, name) for oid, name in wl.objects]
+    pids = [f"{o.id}.{p}" for o in objs for p in o.params]
+    wl.make_data(gpu_render_cb(pids), pids, mag_keys=("source.mag",))
+    ctx = Context(wl.spec(), max_batch=1024)
+    yield wl, pids, ctx
+    ctx.close()
+
+
+def results(ctx):
+    n = ctx.npars()
+    mean, sigma, ml, mp = (c_double * n)(), (c_double * n)(), (c_double * n)(), (c_double * n)()
+    lz, lze, mx = c_double(), c_double(), c_double()
+    assert L.load().lensed_results(ctx.ptr, mean, sigma, ml, mp, byref(lz), byref(lze), byref(mx)) == 0
+    return np.array(mean), np.array(sigma), lz.value, lze.value, mx.value
+
+
+def test_plain_and_batched_nested_runs_agree(c1_ctx):
+    wl, pids, ctx = c1_ctx
+    lib = L.load()
+    npar, ndim = ctx.npars(), ctx.ndims()
+    pmap = ctx.pmap()
+    print(f"C1 nested: ndim {ndim} npar {npar}", flush=True)
+
+    t0 = time.perf_counter()
+    nested.run_multinest_abi(ctypes.cast(lib.loglike, L.LOGLIKE_FN), ndim, npar,
+                             ctypes.cast(lib.dumper, L.DUMPER_FN), nlive=200, tol=0.5, efr=0.8, ceff=False,
+                             updint=1000, seed=21, context=ctx.ptr)
+    t_plain = time.perf_counter() - t0
+    mean_p, sig_p, logz_p, logz_err_p, maxl_p = results(ctx)
+    print(f"plain run(): {t_plain:.1f} s, logZ {logz_p:.3f} +- {logz_err_p:.3f}, max logL {maxl_p:.2f}", flush=True)
+
+    t0 = time.perf_counter()
+    rb = nested.run_context(ctx, 0, nlive=200, tol=0.5, efr=0.8, batch=1024, seed=22)
+    t_batch = time.perf_counter() - t0
+    print(f"batched: {t_batch:.1f} s, {rb}", flush=True)
+    mean_b = np.zeros(npar)
+    sig_b = np.zeros(npar)
+    mean_b[pmap] = rb.mean
+    sig_b[pmap] = rb.sigma
+
+    err = math.hypot(logz_err_p, rb.logz_err)
+    assert abs(logz_p - rb.logz) < 4 * err + 0.1, (logz_p, rb.logz, err)
+    dev = np.abs(mean_p - mean_b) / sig_p
+    print("mean difference / sigma:", dict(zip(pids, np.round(dev, 3))), flush=True)
+    assert np.all(dev < 0.3), dev
+    assert np.all(np.abs(sig_b / sig_p - 1) < 0.3)
+    truth = wl.truth_vector(pids)
+    assert np.all(np.abs(mean_b - truth) < 5 * sig_b + 1e-6), (mean_b - truth) / sig_b
+    assert abs(rb.max_loglike - maxl_p) < 5
