@@ -232,8 +232,9 @@ int lensed_emulator_stats(const lensed* l, long* fallbacks, long* checked, doubl
 /* src/lensed.c:1270-1275 -- MultiNest's C entry point, with the arguments
  * Lensed passes.  Provided here by a single-ellipsoid nested sampler that
  * calls LogLike once per candidate and Dumper every updInt iterations; IS,
- * mmodal, nClsPar, maxModes, Ztol, fb, initMPI and logZero are accepted and
- * ignored (no importance weighting, mode separation or MPI).  efr is the
+ * mmodal, nClsPar, maxModes, Ztol, initMPI and logZero are accepted and
+ * ignored (no importance weighting, mode separation or MPI); fb = 1 prints
+ * progress on stderr.  efr is the
  * target acceptance when ceff is set, else the shrinking factor.  With a
  * root and resume = 1 the run continues from <root>resume.dat; outfile = 1
  * writes <root>.txt (weight, -2 logL, parameters per posterior sample). */
@@ -260,6 +261,7 @@ typedef struct {
     int outfile;     /* write <root>.txt */
     const int* wrap; /* npar periodic flags (sampler order), or NULL */
     const char* root;/* output prefix, or NULL */
+    int feedback;    /* progress lines on stderr every updint (or 1000) iterations */
 } lensed_ns_options;
 
 typedef struct {

@@ -39,7 +39,7 @@ class Options(Structure):
 class NsOptions(Structure):
     _fields_ = [("nlive", c_int), ("tol", c_double), ("efr", c_double), ("ceff", c_int), ("batch", c_int),
                 ("seed", c_int), ("updint", c_int), ("maxiter", c_int), ("resume", c_int), ("outfile", c_int),
-                ("wrap", POINTER(c_int)), ("root", c_char_p)]
+                ("wrap", POINTER(c_int)), ("root", c_char_p), ("feedback", c_int)]
 
 
 class NsResult(Structure):

@@ -7,13 +7,14 @@
 // the same entry point and callbacks, and a batched variant that proposes K
 // points at a time so one launch evaluates them all.
 //
-// Algorithm (Skilling 2004; single bounding ellipsoid as in Mukherjee,
-// Parkinson & Liddle 2006): nlive points uniform in the unit cube; at each
-// iteration the lowest-likelihood point dies with prior-volume shrinkage
-// X_i = exp(-i / nlive) and is replaced by a draw from the prior restricted
-// to L > L_min.  Draws come from the ellipsoid bounding the live points
-// (covariance-shaped, scaled to contain them all, enlarged), rejected where
-// L <= L_min or outside the cube (wrapped for periodic parameters).  Proposals
+// Algorithm (Skilling 2004; ellipsoidal bounds as in Mukherjee, Parkinson &
+// Liddle 2006 and Feroz, Hobson & Bridges 2009): nlive points uniform in the
+// unit cube; at each iteration the lowest-likelihood point dies with
+// prior-volume shrinkage X_i = exp(-i / nlive) and is replaced by a draw from
+// the prior restricted to L > L_min.  Draws come from a union of ellipsoids
+// bounding the live points (covariance-shaped, scaled to contain them all,
+// enlarged), rejected where L <= L_min or outside the cube (wrapped for
+// periodic parameters).  Proposals
 // are drawn in batches; a batch is consumed in order against the rising
 // threshold, which keeps every accepted point a uniform draw from the
 // constrained prior as long as the ellipsoid still bounds it.  The evidence
@@ -25,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <random>
 #include <string>
 #include <vector>
@@ -67,21 +69,206 @@ bool cholesky(std::vector<double>& a, int n) {
     return true;
 }
 
+double log_unit_ball(int n) { return 0.5 * n * std::log(M_PI) - std::lgamma(0.5 * n + 1); }
+
+// An ellipsoid {x : |L^-1 (x - mean)| <= scale} with L the Cholesky factor
+// of the covariance of the points it was fitted to.
 struct Ellipsoid {
     int n = 0;
-    std::vector<double> mean, chol;  // x = mean + scale * chol * z, |z| <= 1
-    std::vector<double> shift;       // periodic coordinates are fitted as frac(u + shift)
+    std::vector<double> mean, chol;
     double scale = 1;
+    double logvol = 0;  // log volume
 
-    double coord(const Point& p, int i) const {
-        double v = p.u[i] + shift[i];
-        return shift[i] != 0 ? v - std::floor(v) : v;
+    // Mahalanobis radius^2 of x (in units of the covariance, before scale)
+    double radius2(const double* x) const {
+        double r2 = 0;
+        double y[64];
+        for (int i = 0; i < n; ++i) {
+            double t = x[i] - mean[i];
+            for (int k = 0; k < i; ++k) t -= chol[i * n + k] * y[k];
+            y[i] = t / chol[i * n + i];
+            r2 += y[i] * y[i];
+        }
+        return r2;
     }
 
-    // bound the points, then enlarge linearly by `enlarge`; a periodic
-    // coordinate is first rotated so that the circular mean of the points
-    // sits at 0.5, which keeps a mode across the 0/1 seam in one piece
-    bool fit(const std::vector<Point>& pts, double enlarge, const std::vector<int>& wrap) {
+    bool contains(const double* x) const { return radius2(x) <= scale * scale; }
+
+    // bound the m points xs[m][n], then multiply the volume by `volfac`;
+    // with boot > 0 the bound is further expanded by the bootstrap (fit on a
+    // resample, measure how far the points left out stick out, keep the
+    // largest ratio over `boot` resamples), which a piece of a region split
+    // off from the rest needs, since its points' hull undercovers its share
+    bool fit(const std::vector<const double*>& xs, int dim, double volfac, int boot = 0,
+             std::mt19937_64* rng = nullptr) {
+        n = dim;
+        int m = (int)xs.size();
+        mean.assign(n, 0.0);
+        for (const double* x : xs)
+            for (int i = 0; i < n; ++i) mean[i] += x[i] / m;
+        std::vector<double> cov((size_t)n * n, 0.0);
+        for (const double* x : xs)
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j <= i; ++j) cov[i * n + j] += (x[i] - mean[i]) * (x[j] - mean[j]) / (m - 1);
+        for (int i = 0; i < n; ++i) {
+            cov[i * n + i] += 1e-12;
+            for (int j = 0; j < i; ++j) cov[j * n + i] = cov[i * n + j];
+        }
+        chol = cov;
+        if (m <= n || !cholesky(chol, n)) return false;
+        double rmax = 0;
+        for (const double* x : xs) rmax = std::max(rmax, radius2(x));
+        double expand = 1;
+        for (int b = 0; b < boot && rng; ++b) {
+            std::uniform_int_distribution<int> pick(0, m - 1);
+            std::vector<char> in(m, 0);
+            std::vector<const double*> sub(m);
+            for (int k = 0; k < m; ++k) {
+                int j = pick(*rng);
+                sub[k] = xs[j];
+                in[j] = 1;
+            }
+            Ellipsoid e;
+            if (!e.fit(sub, n, 1.0)) continue;
+            double far = 0;
+            for (int k = 0; k < m; ++k)
+                if (!in[k]) far = std::max(far, std::sqrt(e.radius2(xs[k])) / e.scale);
+            expand = std::max(expand, far);
+        }
+        scale = std::sqrt(rmax) * expand * std::pow(volfac, 1.0 / n);
+        logvol = log_unit_ball(n) + n * std::log(scale);
+        for (int i = 0; i < n; ++i) logvol += std::log(chol[i * n + i]);
+        return true;
+    }
+
+    // the sphere around the whole unit cube
+    void cube(int dim) {
+        n = dim;
+        mean.assign(n, 0.5);
+        chol.assign((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i) chol[i * n + i] = 1;
+        scale = 0.5 * std::sqrt((double)n) * 1.0001;
+        logvol = log_unit_ball(n) + n * std::log(scale);
+    }
+
+    void sample(std::mt19937_64& rng, double* out) const {
+        std::normal_distribution<double> g(0, 1);
+        std::uniform_real_distribution<double> u(0, 1);
+        double z[64];
+        double r2 = 0;
+        for (int i = 0; i < n; ++i) {
+            z[i] = g(rng);
+            r2 += z[i] * z[i];
+        }
+        double rad = std::pow(u(rng), 1.0 / n) / std::sqrt(r2);
+        for (int i = 0; i < n; ++i) {
+            double s = 0;
+            for (int k = 0; k <= i; ++k) s += chol[i * n + k] * z[k];
+            out[i] = mean[i] + scale * rad * s;
+        }
+    }
+};
+
+// The union of ellipsoids bounding the live points (Feroz, Hobson & Bridges
+// 2009, MultiNest's bound): the set is split recursively by 2-means while
+// the two halves' ellipsoids take less than half the volume of their parent,
+// or while the parent is more than twice the volume its points are expected
+// to occupy (their share m/N of the remaining prior volume X, enlarged).
+// Periodic coordinates are first rotated so that the circular mean of the
+// points sits at 0.5, which keeps a mode across the 0/1 seam in one piece.
+struct Bound {
+    static constexpr int kBoot = 5;  // bootstrap resamples of a split-off piece
+    int n = 0;
+    int boot = kBoot;           // 0 in constant efficiency mode
+    std::vector<double> shift;  // x = frac(u + shift) for periodic coordinates
+    std::vector<Ellipsoid> ells;
+    std::vector<double> cum;    // cumulative volume fractions
+    double logvol_union = 0;    // log of the summed volumes (bounds the union's)
+
+    void to_x(const double* u, double* x) const {
+        for (int i = 0; i < n; ++i) {
+            double v = u[i] + shift[i];
+            x[i] = shift[i] != 0 ? v - std::floor(v) : v;
+        }
+    }
+
+    static double logsum(const std::vector<Ellipsoid>& es) {
+        double top = -INFINITY, acc = 0;
+        for (const auto& e : es) top = std::max(top, e.logvol);
+        for (const auto& e : es) acc += std::exp(e.logvol - top);
+        return top + std::log(acc);
+    }
+
+    // the leaves that bound xs: the parent itself, or the pieces of a
+    // recursive split when they take less volume than the parent
+    std::vector<Ellipsoid> split(const std::vector<const double*>& xs, const Ellipsoid& parent, double volfac,
+                                 int nlive, double logx, std::mt19937_64& rng, int depth) {
+        const int m = (int)xs.size(), minpts = 2 * (n + 1);
+        if (m < 2 * minpts || depth > 8) return {parent};
+        // 2-means, started from a random point's farthest point and that one's farthest
+        std::vector<double> ca(n), cb(n);
+        {
+            auto far = [&](const double* c) {
+                int best = 0;
+                double dmax = -1;
+                for (int k = 0; k < m; ++k) {
+                    double d = 0;
+                    for (int i = 0; i < n; ++i) d += (xs[k][i] - c[i]) * (xs[k][i] - c[i]);
+                    if (d > dmax) dmax = d, best = k;
+                }
+                return best;
+            };
+            int a = far(xs[std::uniform_int_distribution<int>(0, m - 1)(rng)]);
+            int b = far(xs[a]);
+            ca.assign(xs[a], xs[a] + n);
+            cb.assign(xs[b], xs[b] + n);
+        }
+        std::vector<char> side(m, 0);
+        for (int it = 0; it < 20; ++it) {
+            bool changed = false;
+            for (int k = 0; k < m; ++k) {
+                double da = 0, db = 0;
+                for (int i = 0; i < n; ++i) {
+                    da += (xs[k][i] - ca[i]) * (xs[k][i] - ca[i]);
+                    db += (xs[k][i] - cb[i]) * (xs[k][i] - cb[i]);
+                }
+                char s = db < da;
+                changed |= s != side[k];
+                side[k] = s;
+            }
+            std::fill(ca.begin(), ca.end(), 0.0);
+            std::fill(cb.begin(), cb.end(), 0.0);
+            int na = 0, nb = 0;
+            for (int k = 0; k < m; ++k) {
+                auto& c = side[k] ? cb : ca;
+                for (int i = 0; i < n; ++i) c[i] += xs[k][i];
+                (side[k] ? nb : na) += 1;
+            }
+            if (!na || !nb) break;
+            for (int i = 0; i < n; ++i) ca[i] /= na, cb[i] /= nb;
+            if (!changed && it) break;
+        }
+        std::vector<const double*> xa, xb;
+        for (int k = 0; k < m; ++k) (side[k] ? xb : xa).push_back(xs[k]);
+        if ((int)xa.size() < minpts || (int)xb.size() < minpts) return {parent};
+        Ellipsoid ea, eb;
+        if (!ea.fit(xa, n, volfac, boot, &rng) || !eb.fit(xb, n, volfac, boot, &rng)) return {parent};
+        // Look deeper when the halves halve the volume, or when the parent is
+        // more than twice the volume its points are expected to occupy (their
+        // share m/N of the remaining prior volume X, enlarged) -- several
+        // modes or a curved region, whose first cut may not gain yet.
+        double both = logsum({ea, eb});
+        double expected = logx + std::log(volfac * m / nlive);
+        if (!(both < parent.logvol + std::log(0.5)) && !(parent.logvol > expected + std::log(2.0))) return {parent};
+        std::vector<Ellipsoid> leaves = split(xa, ea, volfac, nlive, logx, rng, depth + 1);
+        std::vector<Ellipsoid> more = split(xb, eb, volfac, nlive, logx, rng, depth + 1);
+        leaves.insert(leaves.end(), more.begin(), more.end());
+        if (logsum(leaves) < parent.logvol + std::log(0.8)) return leaves;
+        return {parent};
+    }
+
+    void fit(const std::vector<Point>& pts, double volfac, double logx, const std::vector<int>& wrap, bool multi,
+             std::mt19937_64& rng) {
         n = (int)pts[0].u.size();
         int m = (int)pts.size();
         shift.assign(n, 0.0);
@@ -96,64 +283,50 @@ struct Ellipsoid {
             shift[i] = 0.5 - (c - std::floor(c));
             if (shift[i] == 0) shift[i] = 1;  // still marks the coordinate periodic
         }
-        mean.assign(n, 0.0);
-        for (const auto& p : pts)
-            for (int i = 0; i < n; ++i) mean[i] += coord(p, i) / m;
-        std::vector<double> cov((size_t)n * n, 0.0);
-        for (const auto& p : pts)
-            for (int i = 0; i < n; ++i)
-                for (int j = 0; j <= i; ++j)
-                    cov[i * n + j] += (coord(p, i) - mean[i]) * (coord(p, j) - mean[j]) / (m - 1);
-        for (int i = 0; i < n; ++i) {
-            cov[i * n + i] += 1e-12;
-            for (int j = 0; j < i; ++j) cov[j * n + i] = cov[i * n + j];
+        std::vector<double> xbuf((size_t)m * n);
+        std::vector<const double*> xs(m);
+        for (int k = 0; k < m; ++k) {
+            to_x(pts[k].u.data(), &xbuf[(size_t)k * n]);
+            xs[k] = &xbuf[(size_t)k * n];
         }
-        chol = cov;
-        if (!cholesky(chol, n)) {  // degenerate live set: the sphere around the whole cube
+        ells.clear();
+        Ellipsoid root;
+        if (!root.fit(xs, n, volfac)) {  // degenerate live set
             std::fill(shift.begin(), shift.end(), 0.0);
-            mean.assign(n, 0.5);
-            chol.assign((size_t)n * n, 0.0);
-            for (int i = 0; i < n; ++i) chol[i * n + i] = 1;
-            scale = 0.5 * std::sqrt((double)n);
-            return false;
+            root.cube(n);
+            ells.push_back(root);
+        } else if (multi) {
+            ells = split(xs, root, volfac, m, logx, rng, 0);
+        } else {
+            ells.push_back(root);
         }
-        // largest Mahalanobis radius of the points: solve L y = x - mean
-        double rmax = 0;
-        std::vector<double> y(n);
-        for (const auto& p : pts) {
-            double r2 = 0;
-            for (int i = 0; i < n; ++i) {
-                double t = coord(p, i) - mean[i];
-                for (int k = 0; k < i; ++k) t -= chol[i * n + k] * y[k];
-                y[i] = t / chol[i * n + i];
-                r2 += y[i] * y[i];
-            }
-            rmax = std::max(rmax, r2);
-        }
-        scale = std::sqrt(rmax) * enlarge;
-        return true;
+        cum.assign(ells.size(), 0.0);
+        double top = -INFINITY, acc = 0;
+        for (const auto& e : ells) top = std::max(top, e.logvol);
+        for (size_t k = 0; k < ells.size(); ++k) cum[k] = acc += std::exp(ells[k].logvol - top);
+        for (auto& c : cum) c /= acc;
+        logvol_union = top + std::log(acc);
     }
 
+    // a uniform draw from the union (in unit-cube coordinates, unwrapped)
     void sample(std::mt19937_64& rng, double* out) const {
-        std::normal_distribution<double> g(0, 1);
         std::uniform_real_distribution<double> u(0, 1);
-        std::vector<double> z(n);
-        double r2 = 0;
-        for (int i = 0; i < n; ++i) {
-            z[i] = g(rng);
-            r2 += z[i] * z[i];
+        double x[64];
+        for (;;) {
+            size_t k = std::lower_bound(cum.begin(), cum.end(), u(rng)) - cum.begin();
+            if (k >= ells.size()) k = ells.size() - 1;
+            ells[k].sample(rng, x);
+            int q = 0;  // the union's overlap: accept with 1 / (ellipsoids holding x)
+            for (const auto& e : ells) q += e.contains(x);
+            if (q <= 1 || u(rng) * q < 1) break;
         }
-        double rad = std::pow(u(rng), 1.0 / n) / std::sqrt(r2);
-        for (int i = 0; i < n; ++i) {
-            double s = 0;
-            for (int k = 0; k <= i; ++k) s += chol[i * n + k] * z[k];
-            out[i] = mean[i] + scale * rad * s - shift[i];
-        }
+        for (int i = 0; i < n; ++i) out[i] = x[i] - shift[i];
     }
 };
 
 struct Options {
     int nlive = 300, batch = 1, seed = 0, updint = 1000, maxiter = 0, ceff = 0, resume = 0, outfile = 0;
+    int feedback = 0;
     double tol = 0.1, efr = 0.8;
     std::vector<int> wrap;
     std::string root;
@@ -292,8 +465,8 @@ void call_dumper(DumperFn dumper, void* ctx, Result& r, const std::vector<Point>
 }
 
 int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn dumper, void* dctx, Result& r) {
-    if (ndim <= 0 || o.nlive <= ndim) {
-        lsd::set_error("nested sampling needs 0 < ndim < nlive");
+    if (ndim <= 0 || ndim > 64 || o.nlive <= ndim) {
+        lsd::set_error("nested sampling needs 0 < ndim < nlive and ndim <= 64");
         return -1;
     }
     std::mt19937_64 rng((unsigned long long)o.seed * 6364136223846793005ull + 1442695040888963407ull);
@@ -318,11 +491,13 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
     const double shrink = -1.0 / n;  // log X_i - log X_{i-1}
     double logx = shrink * r.niter;
     // enlargement: without ceff, efr is MultiNest's shrinking factor and the
-    // bounding volume grows by 1/efr (at least 10% in every direction); with
-    // ceff, efr is the target acceptance and the enlargement adapts to it,
-    // never below the tight bound of the live points
-    double enlarge = o.ceff ? 1.2 : std::max(1.1, std::pow(1.0 / o.efr, 1.0 / ndim));
-    Ellipsoid ell;
+    // bounding volume grows by 1/efr (by at least half); with ceff, efr is
+    // the target acceptance and the enlargement adapts to it, never below
+    // the tight bound of the live points
+    double volfac = o.ceff ? 1.5 : std::max(1.5, 1.0 / o.efr);
+    Bound bound;
+    if (o.ceff) bound.boot = 0;
+    long fb_acc = 0, fb_prop = 0;
     std::vector<double> cubes, phys, ll;
     size_t next = 0, have = 0;
     int since_fit = n;
@@ -352,24 +527,30 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
         bool done = false;
         while (!done) {
             if (next >= have) {
-                if (since_fit >= std::max(1, n / 10) || ell.n == 0) {
-                    ell.fit(live, enlarge, o.wrap);
+                if (since_fit >= std::max(1, n / 10) || bound.ells.empty()) {
+                    bound.fit(live, volfac, logx, o.wrap, true, rng);
                     since_fit = 0;
                 }
                 int b = std::max(1, o.batch);
                 cubes.resize((size_t)b * ndim);
                 phys.resize((size_t)b * npar);
                 ll.resize(b);
+                // While the bound is larger than the cube (early on, and always
+                // in many dimensions, where the ellipsoid around a cube holds
+                // the cube as a tiny fraction) the prior itself is the better
+                // proposal; either is a uniform draw over a superset of the
+                // constrained region, which is all the acceptance test needs.
+                const bool use_cube = bound.logvol_union >= 0;
+                double x[64];
                 for (int k = 0; k < b; ++k) {
-                    std::vector<double> x(ndim);
-                    for (int tries = 0;; ++tries) {
-                        ell.sample(rng, x.data());
-                        if (inside(x.data(), ndim, o.wrap, &cubes[(size_t)k * ndim])) break;
-                        if (tries > 100000) {  // ellipsoid far outside the cube: fall back to the prior
-                            for (int i = 0; i < ndim; ++i) cubes[(size_t)k * ndim + i] = uni(rng);
-                            break;
-                        }
+                    double* c = &cubes[(size_t)k * ndim];
+                    bool got = false;
+                    for (int tries = 0; !use_cube && !got && tries < 1000; ++tries) {
+                        bound.sample(rng, x);
+                        got = inside(x, ndim, o.wrap, c);
                     }
+                    if (!got)
+                        for (int i = 0; i < ndim; ++i) c[i] = uni(rng);
                 }
                 fn(cubes.data(), b, phys.data(), ll.data(), ctx);
                 r.nevals += b;
@@ -388,17 +569,35 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
             }
         }
         // constant efficiency mode: steer the enlargement towards acceptance efr
-        if (o.ceff && proposed >= 100) {
-            double acc = (double)accepted / proposed;
-            enlarge *= acc < o.efr ? 0.98 : 1.02;
-            enlarge = std::min(4.0, std::max(1.0, enlarge));
+        // (as in MultiNest, the bound may then shrink below the one that
+        // encloses the live points, which trades evidence accuracy for speed)
+        if (proposed >= 100) {
+            if (o.ceff) {
+                double acc = (double)accepted / proposed;
+                volfac *= acc < o.efr ? 0.9 : 1.1;
+                volfac = std::min(1e3, std::max(1e-2, volfac));
+            }
+            fb_acc += accepted;
+            fb_prop += proposed;
             accepted = proposed = 0;
+        }
+        if (o.feedback && r.niter % (o.updint > 0 ? o.updint : 1000) == 0)
+        {
+            fprintf(stderr,
+                    "nested: iteration %ld  ln Z %.4f  ln X %.2f  evaluations %ld  ellipsoids %zu  ln V %.2f  "
+                    "acceptance %.2e%s\n",
+                    r.niter, r.logz, logx, r.nevals, bound.ells.size(), bound.logvol_union,
+                    fb_prop ? (double)fb_acc / fb_prop : 0.0, bound.logvol_union >= 0 ? "  (prior draws)" : "");
+            fb_acc = fb_prop = 0;
         }
         if (o.updint > 0 && r.niter % o.updint == 0) {
             call_dumper(dumper, dctx, r, live, logx, npar);
             if (!ckpt.empty()) write_checkpoint(ckpt, live, r, ndim, npar);
         }
     }
+    // the checkpoint keeps the state of the iteration (a resumed run goes on
+    // from here), before the live points are added to the evidence
+    if (!ckpt.empty()) write_checkpoint(ckpt, live, r, ndim, npar);
     // remaining live points share the last volume
     for (const auto& p : live) {
         double lw = logx - std::log((double)n);
@@ -408,7 +607,6 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
     }
     r.logz_err = std::sqrt(std::max(r.info, 0.0) / n);
     summarise(r, live, logx, npar);
-    if (!ckpt.empty()) write_checkpoint(ckpt, live, r, ndim, npar);
     if (o.outfile && !o.root.empty()) {
         FILE* f = fopen((o.root + ".txt").c_str(), "w");
         if (f) {  // MultiNest's <root>.txt: weight, -2 logL, parameters
@@ -467,6 +665,7 @@ Options to_options(const lensed_ns_options* in, int npar) {
     o.maxiter = in->maxiter;
     o.resume = in->resume;
     o.outfile = in->outfile;
+    o.feedback = in->feedback;
     if (in->wrap) o.wrap.assign(in->wrap, in->wrap + npar);
     if (in->root) o.root = in->root;
     return o;
@@ -517,7 +716,7 @@ void run(int IS, int mmodal, int ceff, int nlive, double tol, double efr, int nd
          int initMPI, double logZero, int maxiter, void (*LogLike)(double*, int*, int*, double*, void*),
          void (*Dumper)(int*, int*, int*, double**, double**, double**, double*, double*, double*, double*, void*),
          void* context) {
-    (void)IS; (void)mmodal; (void)nClsPar; (void)maxModes; (void)Ztol; (void)fb; (void)initMPI; (void)logZero;
+    (void)IS; (void)mmodal; (void)nClsPar; (void)maxModes; (void)Ztol; (void)initMPI; (void)logZero;
     Options o;
     o.nlive = nlive;
     o.tol = tol;
@@ -528,6 +727,7 @@ void run(int IS, int mmodal, int ceff, int nlive, double tol, double efr, int nd
     o.maxiter = maxiter;
     o.resume = resume;
     o.outfile = outfile;
+    o.feedback = fb;
     o.batch = 1;
     if (pWrap) o.wrap.assign(pWrap, pWrap + nPar);
     if (root) o.root = root;

@@ -3,7 +3,7 @@
 Lensed runs MultiNest on its `loglike` callback (src/lensed.c:1236-1288).
 liblensed_amd.so carries a nested sampler behind the same `run()` entry
 point, plus a batched form that proposes many candidates per likelihood call
-so one launch evaluates them all (`Context.nested`).  This module exposes
+so one launch evaluates them all (`run_context`).  This module exposes
 both to Python; the sampling itself is native code.
 """
 import ctypes
@@ -34,9 +34,9 @@ class NestedResult:
 
 
 def options(nlive=300, tol=0.1, efr=0.8, ceff=False, batch=1, seed=0, updint=0, maxiter=0, resume=False,
-            outfile=False, wrap=None, root=None):
+            outfile=False, wrap=None, root=None, feedback=False):
     o = L.NsOptions(nlive=nlive, tol=tol, efr=efr, ceff=int(ceff), batch=batch, seed=seed, updint=updint,
-                    maxiter=maxiter, resume=int(resume), outfile=int(outfile))
+                    maxiter=maxiter, resume=int(resume), outfile=int(outfile), feedback=int(feedback))
     keep = []
     if wrap is not None:
         w = (c_int * len(wrap))(*[int(bool(v)) for v in wrap])
@@ -83,8 +83,6 @@ def run_context(ctx, s=0, **kw):
     """Batched nested sampling of system s of a lensed_amd.lensed.Context."""
     lib = L.load()
     npar = ctx.npars(s)
-    if kw.get("wrap") is None:
-        kw["wrap"] = None
     o, keep = options(**kw)
     mean, sigma, ml = (c_double * npar)(), (c_double * npar)(), (c_double * npar)()
     res = L.NsResult()
@@ -96,7 +94,8 @@ def run_context(ctx, s=0, **kw):
 
 
 def run_multinest_abi(loglike, ndim, npar, dumper=None, nlive=300, tol=0.1, efr=0.8, ceff=False, updint=100,
-                      seed=0, wrap=None, root=None, resume=False, outfile=False, maxiter=0, context=None):
+                      seed=0, wrap=None, root=None, resume=False, outfile=False, maxiter=0, context=None,
+                      feedback=False):
     """Call the library's MultiNest-compatible run() with ctypes callbacks.
 
     loglike(cube, ndim, npar, lnew, ctx) and dumper(...) have MultiNest's C
@@ -110,5 +109,5 @@ def run_multinest_abi(loglike, ndim, npar, dumper=None, nlive=300, tol=0.1, efr=
         dfn = dumper if isinstance(dumper, L.DUMPER_FN) else L.DUMPER_FN(dumper)
     w = (c_int * npar)(*([0] * npar if wrap is None else [int(bool(v)) for v in wrap]))
     rootbuf = ctypes.create_string_buffer((root or "").encode(), 100)
-    lib.run(1, 1, int(ceff), nlive, tol, efr, ndim, npar, ndim, 100, updint, -1e90, rootbuf, seed, w, 0,
+    lib.run(1, 1, int(ceff), nlive, tol, efr, ndim, npar, ndim, 100, updint, -1e90, rootbuf, seed, w, int(feedback),
             int(resume), int(outfile), 1, -np.finfo(np.float64).max, maxiter, lfn, dfn, context)

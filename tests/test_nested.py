@@ -72,6 +72,37 @@ def test_periodic_parameter_across_the_seam():
     assert abs(r.logz - 0.5 * math.log(2 * math.pi * s * s)) < 4 * r.logz_err + 0.05
 
 
+def grid_logz(logl, n=2000):
+    """ln of the mean of L over the unit square, by the midpoint rule."""
+    g = (np.arange(n) + 0.5) / n
+    x, y = np.meshgrid(g, g)
+    ll = logl(np.stack([x.ravel(), y.ravel()], 1))
+    return ll.max() + math.log(np.mean(np.exp(ll - ll.max())))
+
+
+def eggbox(u):
+    x = u * 10 * np.pi
+    return (2 + np.cos(x[:, 0] / 2) * np.cos(x[:, 1] / 2)) ** 5
+
+
+def two_shells(u):
+    x = u * 12 - 6
+    out = [-0.5 * ((np.hypot(x[:, 0] - c, x[:, 1]) - 2) / 0.1) ** 2 - 0.5 * math.log(2 * math.pi * 0.01)
+           for c in (-3.5, 3.5)]
+    return np.logaddexp(out[0], out[1])
+
+
+@pytest.mark.parametrize("logl", [eggbox, two_shells], ids=["eggbox", "two_shells"])
+def test_multimodal_and_curved_evidence(logl):
+    """Feroz, Hobson & Bridges (2009) test problems: 18 modes, and two thin
+    rings; the evidence against brute-force integration, and the bound made of
+    several ellipsoids keeps the cost per iteration low."""
+    ref = grid_logz(logl)
+    r = nested.sample(logl, 2, nlive=500, tol=0.1, batch=256, seed=4)
+    assert abs(r.logz - ref) < 4 * r.logz_err + 0.05, (r.logz, ref, r.logz_err)
+    assert r.nevals / r.niter < 20
+
+
 def test_multinest_abi_run_and_dumper():
     """run() with MultiNest's callbacks: cube is unit in, physical out."""
     ndim, npar = 2, 3
