@@ -38,6 +38,10 @@ FLOP_CHI2 = 4 * 65536
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X published FP32 (packed / matrix) peak
 PEAK_FP32_NONPACKED = 78.6
 PEAK_HBM = 8000.0  # GB/s
+# HBM bytes per 4096-candidate render launch, measured with rocprofv3 --pmc:
+# FETCH_SIZE 137480 KiB (x 2, the factor a 1 GiB copy calibrates) in and
+# WRITE_SIZE 1048576 KiB (x 1) out, i.e. the 4096 model images
+TRAFFIC_RENDER_4096 = (137480 * 2 + 1048576) * 1024
 
 
 def make_workload(K):
@@ -222,6 +226,19 @@ def nested_compare(wl, pids, K, nlive=400):
                              "log-evidence is not an accurate estimate in either run"}
 
 
+def copy_probe(mib, reps=3):
+    """Plain device-to-device copies of a known size: what the profiler's
+    byte counters read for them calibrates the counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "copies": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +248,10 @@ def main():
     ap.add_argument("--plain", type=int, default=1000, help="candidates timed on the plain path")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-nested", action="store_true", help="skip the end-to-end nested-sampling comparison")
+    ap.add_argument("--value-only", action="store_true",
+                    help="time only the batched + emulated launches and the roofline (for profiler runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (byte-counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: candidates are independent, so the path shards by splitting a "
@@ -238,9 +259,29 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     from lensed_amd._lib import LENSED_EMULATE, LENSED_GRAPH, LENSED_TABLE
     K = args.batch
     wl, pids = make_workload(K)
+    if args.value_only:
+        dt, _, emu_stats = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_EMULATE)
+        kt = kernel_times(wl, K, 3, LENSED_EMULATE)
+        achieved = FLOP_RENDER * K / (kt["render"] / 1e3) / 1e12
+        print(json.dumps({"metric": METRIC, "value": round(args.steps * K / dt, 2), "unit": "logL evals/s",
+                          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": "C3 (see the full run)", "candidates_per_launch": K,
+                                     "legs": "batched + emulated only"},
+                          "roofline": {"bound": "valu", "kernel": "lensed_render", "achieved": round(achieved, 3),
+                                       "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
+                                       "traffic": int(TRAFFIC_RENDER_4096 * K / 4096),
+                                       "avg_launch_ms": round(kt["render"], 4)},
+                          "stages_ms_per_launch": {k: round(v, 4) for k, v in kt.items()},
+                          "cpu_baseline": None}))
+        return
 
     dt_exact, lnew, _ = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH)
     exact_rate = args.steps * K / dt_exact
@@ -298,7 +339,8 @@ def main():
         "roofline": {"bound": "valu", "kernel": "lensed_render", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
                      "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED, 4),
-                     "traffic": None, "algorithmic_flop_per_launch": FLOP_RENDER * K,
+                     "traffic": int(TRAFFIC_RENDER_4096 * K / 4096), "traffic_unit": "bytes per launch (rocprofv3 --pmc)",
+                     "algorithmic_flop_per_launch": FLOP_RENDER * K,
                      "avg_launch_ms": round(kt["render"], 4)},
         "stages_ms_per_launch": {k: round(v, 4) for k, v in kt.items()},
         "stages_note": "set_params includes the emulator, held-out check and fallback table kernels",
