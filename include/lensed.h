@@ -280,6 +280,66 @@ int lensed_nested_sample(int ndim, int npar, lensed_batch_fn fn, void* ctx, cons
 int lensed_nested_run(lensed* l, size_t s, const lensed_ns_options* opts, double* mean, double* sigma,
                       double* ml, lensed_ns_result* result);
 
+/* ----------------------------------------------- input data and output */
+
+/* src/data.c:54-118 -- the 2-D image of a FITS file specification in
+ * CFITSIO's syntax (`file.fits`, `file.fits[ext]`, `file.fits[IMG]`,
+ * `file.fits[x1:x2,y1:y2]`, steps `x1:x2:dx`, reversed ranges, `*`), as
+ * floats (BSCALE/BZERO applied), row-major; *data is freed with
+ * lensed_fits_free().  pcs (may be NULL) receives the pixel coordinate
+ * system of src/data.c:236-278: origin (rx, ry) and step (sx, sy). */
+int lensed_fits_read(const char* spec, size_t* width, size_t* height, float** data, double* pcs);
+void lensed_fits_free(void* p);
+/* src/data.c:236-278 -- only the pixel coordinate system of a specification */
+int lensed_read_pcs(const char* spec, double* pcs);
+/* src/data.c:120-165,372-391 -- n float images as IMAGE extensions named
+ * names[k] after an empty primary HDU; a leading '!' is accepted */
+int lensed_fits_write(const char* path, size_t width, size_t height, size_t n, const float* const* images,
+                      const char* const* names);
+/* src/data.c:421-501 -- mode and FWHM of the pixel values (mask: skip zeros) */
+void lensed_find_mode(size_t n, const float* values, const float* mask, double* mode, double* fwhm);
+
+/* the data options of src/input/options.c:63-146; weight, gain and xweight
+ * take a FITS specification or a number, as src/input/options.c:506-532 */
+typedef struct {
+    const char* image;   /* required */
+    const char* weight;  /* weight map or value; else weights = gain / (image + offset) */
+    const char* gain;
+    double offset;
+    double bscale;       /* 0 = none */
+    const char* xweight; /* extra weight multiplier, or NULL */
+    const char* mask;    /* non-zero pixels are masked (weight 0), or NULL */
+    const char* psf;     /* normalised to unit sum, or NULL */
+} lensed_data_options;
+
+#define LENSED_WARN_SKY_NOT_ZERO 1    /* background mode far from zero */
+#define LENSED_WARN_NO_OFFSET 2       /* mode near zero but no offset given */
+#define LENSED_WARN_NEGATIVE_WEIGHT 4 /* some weights not >= 0 */
+
+typedef struct {
+    size_t width, height;
+    float* image;
+    float* weight;
+    float* psf;          /* NULL without a PSF */
+    size_t psfw, psfh;
+    double pcs[4];
+    size_t masked;       /* pixels with weight 0 */
+    double mode, fwhm;   /* background check of src/lensed.c:501-540 */
+    int warnings;        /* LENSED_WARN_* */
+} lensed_data;
+
+/* src/lensed.c:406-549 -- read and prepare one observation */
+int lensed_data_read(const lensed_data_options* opts, lensed_data* data);
+void lensed_data_free(lensed_data* data);
+
+/* src/lensed.c:572-594, src/nested.c:169-226 -- every dumper() call writes
+ * the layers IMG (model), RES (data - model), RAW (unconvolved model), ERR
+ * (quadrature error / model), WHT (weights), PVL (erfc(sqrt(chi^2 / 2)))
+ * of system 0 at the maximum-likelihood point to this FITS file (NULL: off) */
+int lensed_set_output(lensed* l, const char* path);
+/* the same layers for one physical parameter vector (object order) */
+int lensed_write_layers(lensed* l, size_t s, const double* phys, const char* path);
+
 /* results recorded by dumper(): mean, sigma, ML, MAP per parameter in object
  * order (each may be NULL) and the log-evidence summary */
 int lensed_results(const lensed* l, double* mean, double* sigma, double* ml, double* map,

@@ -53,6 +53,20 @@ DUMPER_FN = ctypes.CFUNCTYPE(None, POINTER(c_int), POINTER(c_int), POINTER(c_int
                              POINTER(POINTER(c_double)), POINTER(POINTER(c_double)), POINTER(c_double),
                              POINTER(c_double), POINTER(c_double), POINTER(c_double), c_void_p)
 
+class DataOptions(Structure):
+    _fields_ = [("image", c_char_p), ("weight", c_char_p), ("gain", c_char_p), ("offset", c_double),
+                ("bscale", c_double), ("xweight", c_char_p), ("mask", c_char_p), ("psf", c_char_p)]
+
+
+class Data(Structure):
+    _fields_ = [("width", c_size_t), ("height", c_size_t), ("image", POINTER(c_float)),
+                ("weight", POINTER(c_float)), ("psf", POINTER(c_float)), ("psfw", c_size_t), ("psfh", c_size_t),
+                ("pcs", c_double * 4), ("masked", c_size_t), ("mode", c_double), ("fwhm", c_double),
+                ("warnings", c_int)]
+
+
+LENSED_WARN_SKY_NOT_ZERO, LENSED_WARN_NO_OFFSET, LENSED_WARN_NEGATIVE_WEIGHT = 1, 2, 4
+
 # every entry point declared in include/lensed.h: name -> (restype, argtypes)
 SIGNATURES = {
     "object_program": (None, [c_char_p, POINTER(c_size_t), POINTER(POINTER(c_char_p))]),
@@ -100,6 +114,18 @@ SIGNATURES = {
     "lensed_emulator_eval": (c_int, [POINTER(c_float), POINTER(c_float), POINTER(c_float), c_int, c_int,
                                      POINTER(c_float), POINTER(c_float)]),
     "lensed_emulator_stats": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
+    "lensed_fits_read": (c_int, [c_char_p, POINTER(c_size_t), POINTER(c_size_t), POINTER(POINTER(c_float)),
+                                 POINTER(c_double)]),
+    "lensed_fits_free": (None, [c_void_p]),
+    "lensed_read_pcs": (c_int, [c_char_p, POINTER(c_double)]),
+    "lensed_fits_write": (c_int, [c_char_p, c_size_t, c_size_t, c_size_t, POINTER(POINTER(c_float)),
+                                  POINTER(c_char_p)]),
+    "lensed_find_mode": (None, [c_size_t, POINTER(c_float), POINTER(c_float), POINTER(c_double),
+                                POINTER(c_double)]),
+    "lensed_data_read": (c_int, [POINTER(DataOptions), POINTER(Data)]),
+    "lensed_data_free": (None, [POINTER(Data)]),
+    "lensed_set_output": (c_int, [c_void_p, c_char_p]),
+    "lensed_write_layers": (c_int, [c_void_p, c_size_t, POINTER(c_double), c_char_p]),
     "run": (None, [c_int, c_int, c_int, c_int, c_double, c_double, c_int, c_int, c_int, c_int, c_int, c_double,
                    c_char_p, c_int, POINTER(c_int), c_int, c_int, c_int, c_int, c_double, c_int, LOGLIKE_FN,
                    DUMPER_FN, c_void_p]),

@@ -50,6 +50,7 @@ struct System {
     // EPL angular table: index of the EPL object and of its q, t parameters
     int epl_obj = -1, epl_q = -1, epl_t = -1;
     bool table = false;
+    std::vector<float> h_image, h_weight;  // host copies for the output layers
     // dumper results (object order)
     std::vector<double> mean, sigma, ml, map;
     double logev = 0, logev_err = 0, logev_ins = 0, max_loglike = 0;
@@ -108,6 +109,7 @@ struct lensed {
     size_t cap_params = 0, cap_blocks = 0, cap_model = 0, cap_partials = 0, cap_out = 0, cap_tables = 0;
     std::map<std::vector<size_t>, lsd::Layout> layouts;
     std::string source;          // generated program (for dumps)
+    std::string output;          // FITS file the dumper writes (empty: none)
     // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem
     hipEvent_t ev[9] = {};
     double prof_ms[8] = {};

@@ -1,0 +1,625 @@
+// Input data and output layers: the FITS subset Lensed reads and writes,
+// and the preparation of image, weights, mask and PSF.
+//
+// Lensed reads its inputs through CFITSIO (src/data.c:54-118) and writes the
+// dumper's layers as a multi-extension FITS file (src/data.c:120-165,
+// 372-391).  CFITSIO is not part of this build; this file reads and writes
+// the part of the FITS standard those files use: a primary HDU and IMAGE
+// extensions, BITPIX 8/16/32/64/-32/-64 with BSCALE/BZERO, big-endian data
+// in 2880-byte records, and CFITSIO's file-name syntax for an extension
+// (`file.fits[2]`, `file.fits[IMG]`) and an image section
+// (`file.fits[x1:x2,y1:y2]`, with steps `x1:x2:dx`, reversed ranges and `*`).
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "../../include/lensed.h"
+
+namespace lsd {
+void set_error(const std::string& msg);
+}
+
+namespace {
+
+constexpr size_t kRecord = 2880;
+
+struct Hdu {
+    std::vector<std::pair<std::string, std::string>> cards;  // keyword, raw value
+    size_t data_offset = 0, data_bytes = 0;
+    int bitpix = 0;
+    std::vector<long> naxes;
+    bool is_image = false;
+
+    const std::string* get(const char* key) const {
+        for (const auto& c : cards)
+            if (c.first == key) return &c.second;
+        return nullptr;
+    }
+    long get_long(const char* key, long def) const {
+        const std::string* v = get(key);
+        return v ? std::strtol(v->c_str(), nullptr, 10) : def;
+    }
+    double get_double(const char* key, double def) const {
+        const std::string* v = get(key);
+        if (!v) return def;
+        std::string t = *v;
+        std::replace(t.begin(), t.end(), 'D', 'E');  // Fortran exponents
+        return std::strtod(t.c_str(), nullptr);
+    }
+    std::string get_string(const char* key) const {
+        const std::string* v = get(key);
+        if (!v) return "";
+        std::string t = *v;
+        size_t a = t.find('\''), b = t.rfind('\'');
+        if (a != std::string::npos && b != std::string::npos && b > a) t = t.substr(a + 1, b - a - 1);
+        while (!t.empty() && t.back() == ' ') t.pop_back();
+        return t;
+    }
+};
+
+std::string trim(const std::string& s) {
+    size_t a = s.find_first_not_of(' '), b = s.find_last_not_of(' ');
+    return a == std::string::npos ? "" : s.substr(a, b - a + 1);
+}
+
+// the value part of a card (columns 11-80), without its comment
+std::string card_value(const char* card) {
+    std::string v(card + 10, 70);
+    if (!v.empty() && trim(v)[0] == '\'') {  // string: up to the closing quote ('' escapes one)
+        size_t a = v.find('\'');
+        size_t i = a + 1;
+        for (; i < v.size(); ++i) {
+            if (v[i] == '\'') {
+                if (i + 1 < v.size() && v[i + 1] == '\'') {
+                    ++i;
+                    continue;
+                }
+                break;
+            }
+        }
+        return v.substr(a, i - a + 1);
+    }
+    size_t slash = v.find('/');
+    return trim(slash == std::string::npos ? v : v.substr(0, slash));
+}
+
+bool parse_hdus(const std::vector<unsigned char>& f, std::vector<Hdu>& hdus, std::string& err) {
+    size_t pos = 0;
+    while (pos + kRecord <= f.size()) {
+        Hdu h;
+        bool end = false;
+        while (!end) {
+            if (pos + kRecord > f.size()) {
+                err = "truncated FITS header";
+                return false;
+            }
+            for (int i = 0; i < 36 && !end; ++i) {
+                const char* card = (const char*)&f[pos + i * 80];
+                std::string key = trim(std::string(card, 8));
+                if (key == "END") end = true;
+                else if (card[8] == '=' && card[9] == ' ') h.cards.emplace_back(key, card_value(card));
+            }
+            pos += kRecord;
+        }
+        if (hdus.empty() && !h.get("SIMPLE")) {
+            err = "not a FITS file (no SIMPLE card)";
+            return false;
+        }
+        h.bitpix = (int)h.get_long("BITPIX", 0);
+        long naxis = h.get_long("NAXIS", 0);
+        size_t n = naxis > 0 ? 1 : 0;
+        for (long i = 1; i <= naxis; ++i) {
+            long v = h.get_long(("NAXIS" + std::to_string(i)).c_str(), 0);
+            h.naxes.push_back(v);
+            n *= (size_t)std::max(0L, v);
+        }
+        std::string xt = h.get_string("XTENSION");
+        h.is_image = hdus.empty() || xt == "IMAGE";
+        long pcount = h.get_long("PCOUNT", 0), gcount = h.get_long("GCOUNT", 1);
+        size_t bytes = (size_t)std::abs(h.bitpix) / 8 * (size_t)gcount * ((size_t)pcount + n);
+        h.data_offset = pos;
+        h.data_bytes = n * (size_t)std::abs(h.bitpix) / 8;
+        pos += (bytes + kRecord - 1) / kRecord * kRecord;
+        hdus.push_back(h);
+    }
+    return true;
+}
+
+// file name, extension and section of a CFITSIO-style file specification
+struct Spec {
+    std::string path, ext, section;
+};
+
+Spec parse_spec(const std::string& s) {
+    Spec sp;
+    size_t b = s.find('[');
+    sp.path = s.substr(0, b);
+    if (!sp.path.empty() && sp.path[0] == '!') sp.path = sp.path.substr(1);
+    while (b != std::string::npos) {
+        size_t e = s.find(']', b);
+        if (e == std::string::npos) break;
+        std::string item = s.substr(b + 1, e - b - 1);
+        if (item.find_first_of(":,*") != std::string::npos) sp.section = item;
+        else sp.ext = item;
+        b = s.find('[', e);
+    }
+    return sp;
+}
+
+struct Range {
+    long first, last, step;  // 1-based, inclusive, step signed
+};
+
+bool parse_range(std::string t, long n, Range& r, std::string& err) {
+    t = trim(t);
+    if (t == "*") {
+        r = {1, n, 1};
+        return true;
+    }
+    if (t == "-*") {
+        r = {n, 1, -1};
+        return true;
+    }
+    long a, c, inc = 1;
+    char* end;
+    a = std::strtol(t.c_str(), &end, 10);
+    if (*end != ':') {
+        err = "invalid image section range: " + t;
+        return false;
+    }
+    c = std::strtol(end + 1, &end, 10);
+    if (*end == ':') inc = std::strtol(end + 1, &end, 10);
+    if (*end || inc <= 0 || a < 1 || c < 1 || a > n || c > n) {
+        err = "invalid image section range: " + t;
+        return false;
+    }
+    r = {a, c, c >= a ? inc : -inc};
+    return true;
+}
+
+template <typename T>
+T be_load(const unsigned char* p) {
+    unsigned char b[sizeof(T)];
+    for (size_t i = 0; i < sizeof(T); ++i) b[i] = p[sizeof(T) - 1 - i];
+    T v;
+    std::memcpy(&v, b, sizeof(T));
+    return v;
+}
+
+double raw_value(const unsigned char* p, int bitpix) {
+    switch (bitpix) {
+        case 8: return p[0];
+        case 16: return be_load<int16_t>(p);
+        case 32: return be_load<int32_t>(p);
+        case 64: return (double)be_load<int64_t>(p);
+        case -32: return be_load<float>(p);
+        case -64: return be_load<double>(p);
+    }
+    return NAN;
+}
+
+bool read_file(const std::string& path, std::vector<unsigned char>& f, std::string& err) {
+    FILE* fp = std::fopen(path.c_str(), "rb");
+    if (!fp) {
+        err = path + ": cannot open file";
+        return false;
+    }
+    std::fseek(fp, 0, SEEK_END);
+    long n = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    f.resize(n > 0 ? (size_t)n : 0);
+    bool ok = n >= 0 && std::fread(f.data(), 1, f.size(), fp) == f.size();
+    std::fclose(fp);
+    if (!ok) err = path + ": read error";
+    return ok;
+}
+
+// the 2-D image of a file specification, as doubles, row-major (x fastest)
+bool read_image(const char* spec_text, size_t& width, size_t& height, std::vector<double>& out, double pcs[4],
+                std::string& err) {
+    Spec sp = parse_spec(spec_text);
+    std::vector<unsigned char> f;
+    if (!read_file(sp.path, f, err)) return false;
+    std::vector<Hdu> hdus;
+    if (!parse_hdus(f, hdus, err)) {
+        err = sp.path + ": " + err;
+        return false;
+    }
+    const Hdu* h = nullptr;
+    if (!sp.ext.empty()) {
+        char* end;
+        long k = std::strtol(sp.ext.c_str(), &end, 10);
+        if (!*end) {
+            if (k >= 0 && k < (long)hdus.size()) h = &hdus[k];
+        } else {
+            std::string want = sp.ext;
+            for (auto& c : want) c = (char)std::toupper((unsigned char)c);
+            for (const auto& x : hdus) {
+                std::string nm = x.get_string("EXTNAME");
+                for (auto& c : nm) c = (char)std::toupper((unsigned char)c);
+                if (nm == want) {
+                    h = &x;
+                    break;
+                }
+            }
+        }
+        if (!h) {
+            err = std::string(spec_text) + ": no such extension";
+            return false;
+        }
+    } else {  // as CFITSIO's fits_open_image: the first HDU that holds an image
+        for (const auto& x : hdus)
+            if (x.is_image && !x.naxes.empty()) {
+                h = &x;
+                break;
+            }
+        if (!h) {
+            err = sp.path + ": no image in file";
+            return false;
+        }
+    }
+    if (!h->is_image) {
+        err = std::string(spec_text) + ": extension is not an image";
+        return false;
+    }
+    if (h->bitpix != 8 && h->bitpix != 16 && h->bitpix != 32 && h->bitpix != 64 && h->bitpix != -32 &&
+        h->bitpix != -64) {
+        err = std::string(spec_text) + ": unsupported BITPIX " + std::to_string(h->bitpix);
+        return false;
+    }
+    if (h->naxes.size() != 2) {
+        err = std::string(spec_text) + ": file has " + std::to_string(h->naxes.size()) + " axes (should be 2)";
+        return false;
+    }
+    if (h->data_offset + h->data_bytes > f.size()) {
+        err = sp.path + ": truncated data";
+        return false;
+    }
+    long nx = h->naxes[0], ny = h->naxes[1];
+    Range rx{1, nx, 1}, ry{1, ny, 1};
+    if (!sp.section.empty()) {
+        size_t comma = sp.section.find(',');
+        if (comma == std::string::npos) {
+            err = std::string(spec_text) + ": image section needs two ranges";
+            return false;
+        }
+        if (!parse_range(sp.section.substr(0, comma), nx, rx, err) ||
+            !parse_range(sp.section.substr(comma + 1), ny, ry, err))
+            return false;
+    }
+    // pixel coordinate system as src/data.c:236-278 derives it from the section
+    pcs[0] = (double)rx.first;
+    pcs[1] = (double)ry.first;
+    pcs[2] = (double)rx.step;
+    pcs[3] = (double)ry.step;
+    width = (size_t)((std::labs(rx.last - rx.first)) / std::labs(rx.step) + 1);
+    height = (size_t)((std::labs(ry.last - ry.first)) / std::labs(ry.step) + 1);
+    double bscale = h->get_double("BSCALE", 1.0), bzero = h->get_double("BZERO", 0.0);
+    size_t bpp = (size_t)std::abs(h->bitpix) / 8;
+    out.resize(width * height);
+    for (size_t j = 0; j < height; ++j) {
+        long y = ry.first + (long)j * ry.step;
+        for (size_t i = 0; i < width; ++i) {
+            long x = rx.first + (long)i * rx.step;
+            size_t k = (size_t)(y - 1) * (size_t)nx + (size_t)(x - 1);
+            out[j * width + i] = bzero + bscale * raw_value(&f[h->data_offset + k * bpp], h->bitpix);
+        }
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ writing
+
+std::string card(const std::string& key, const std::string& value, const std::string& comment) {
+    char buf[81];
+    if (value.empty())
+        std::snprintf(buf, sizeof buf, "%-8s%-72s", key.c_str(), comment.c_str());
+    else if (value[0] == '\'')  // strings start in column 11
+        std::snprintf(buf, sizeof buf, "%-8s= %-20s / %-47s", key.c_str(), value.c_str(), comment.c_str());
+    else  // numbers and logicals right-aligned to column 30
+        std::snprintf(buf, sizeof buf, "%-8s= %20s / %-47s", key.c_str(), value.c_str(), comment.c_str());
+    std::string s(buf);
+    s.resize(80, ' ');
+    return s;
+}
+
+std::string quoted(const std::string& s) {
+    std::string q = "'";
+    for (char c : s) {
+        q += c;
+        if (c == '\'') q += '\'';
+    }
+    while (q.size() < 9) q += ' ';  // at least 8 characters inside the quotes
+    return q + "'";
+}
+
+void pad(std::string& s, char fill) {
+    size_t n = (s.size() + kRecord - 1) / kRecord * kRecord;
+    s.resize(n, fill);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lensed_fits_read(const char* spec, size_t* width, size_t* height, float** data, double* pcs) {
+    if (!spec || !width || !height || !data) {
+        lsd::set_error("lensed_fits_read: null argument");
+        return -1;
+    }
+    std::vector<double> v;
+    double p[4];
+    std::string err;
+    if (!read_image(spec, *width, *height, v, p, err)) {
+        lsd::set_error(err);
+        return -1;
+    }
+    *data = (float*)std::malloc(v.size() * sizeof(float) + 1);
+    if (!*data) {
+        lsd::set_error("out of memory");
+        return -1;
+    }
+    for (size_t i = 0; i < v.size(); ++i) (*data)[i] = (float)v[i];
+    if (pcs) std::memcpy(pcs, p, sizeof p);
+    return 0;
+}
+
+void lensed_fits_free(void* p) { std::free(p); }
+
+int lensed_read_pcs(const char* spec, double* pcs) {
+    size_t w, h;
+    std::vector<double> v;
+    std::string err;
+    if (!read_image(spec, w, h, v, pcs, err)) {
+        lsd::set_error(err);
+        return -1;
+    }
+    return 0;
+}
+
+int lensed_fits_write(const char* path, size_t width, size_t height, size_t n, const float* const* images,
+                      const char* const* names) {
+    if (!path) {
+        lsd::set_error("lensed_fits_write: no file name");
+        return -1;
+    }
+    Spec sp = parse_spec(path);  // a leading '!' (overwrite) is accepted as CFITSIO does
+    std::string out;
+    {
+        char date[32];
+        std::time_t t = std::time(nullptr);
+        std::strftime(date, sizeof date, "%Y-%m-%dT%H:%M:%S", std::gmtime(&t));
+        out += card("SIMPLE", "T", "file does conform to FITS standard");
+        out += card("BITPIX", "16", "number of bits per data pixel");
+        out += card("NAXIS", "0", "number of data axes");
+        out += card("EXTEND", "T", "FITS dataset may contain extensions");
+        out += card("ORIGIN", quoted("lensed-amd"), "FITS file originator");
+        out += card("COMMENT", "", "  layers of the model at the maximum-likelihood parameters");
+        out += card("DATE", quoted(date), "file creation date (YYYY-MM-DDThh:mm:ss UT)");
+        out += card("END", "", "");
+        pad(out, ' ');
+    }
+    for (size_t k = 0; k < n; ++k) {
+        std::string h;
+        h += card("XTENSION", quoted("IMAGE"), "IMAGE extension");
+        h += card("BITPIX", "-32", "number of bits per data pixel");
+        h += card("NAXIS", "2", "number of data axes");
+        h += card("NAXIS1", std::to_string(width), "length of data axis 1");
+        h += card("NAXIS2", std::to_string(height), "length of data axis 2");
+        h += card("PCOUNT", "0", "required keyword; must = 0");
+        h += card("GCOUNT", "1", "required keyword; must = 1");
+        if (names && names[k]) h += card("EXTNAME", quoted(names[k]), "extension name");
+        h += card("END", "", "");
+        pad(h, ' ');
+        out += h;
+        std::string d(width * height * 4, '\0');
+        for (size_t i = 0; i < width * height; ++i) {
+            float v = images[k] ? images[k][i] : 0.0f;
+            unsigned char b[4];
+            std::memcpy(b, &v, 4);
+            for (int j = 0; j < 4; ++j) d[i * 4 + j] = (char)b[3 - j];
+        }
+        pad(d, '\0');
+        out += d;
+    }
+    FILE* fp = std::fopen(sp.path.c_str(), "wb");
+    if (!fp) {
+        lsd::set_error(sp.path + ": cannot create file");
+        return -1;
+    }
+    bool ok = std::fwrite(out.data(), 1, out.size(), fp) == out.size();
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) {
+        lsd::set_error(sp.path + ": write error");
+        return -1;
+    }
+    return 0;
+}
+
+// src/data.c:421-501: mode of the (unmasked) pixel values in MODE_BINS = 100
+// bins between their minimum and maximum, and the full width of the
+// histogram where it falls below half the peak
+void lensed_find_mode(size_t n, const float* values, const float* mask, double* mode, double* fwhm) {
+    const int bins = 100;
+    if (n == 0) {
+        *mode = NAN;
+        *fwhm = 0;
+        return;
+    }
+    double lo = values[0], hi = values[0];
+    for (size_t i = 1; i < n; ++i) {
+        if (mask && !mask[i]) continue;
+        lo = std::min(lo, (double)values[i]);
+        hi = std::max(hi, (double)values[i]);
+    }
+    if (lo == hi) {
+        *mode = lo;
+        *fwhm = 0;
+        return;
+    }
+    double dx = (hi - lo) / bins;
+    std::vector<size_t> counts(bins, 0);
+    for (size_t i = 0; i < n; ++i) {
+        if (mask && !mask[i]) continue;
+        long j = (long)((values[i] - lo) / dx);
+        j = std::min<long>(std::max<long>(j, 0), bins - 1);
+        counts[j] += 1;
+    }
+    int top = 0;
+    for (int i = 1; i < bins; ++i)
+        if (counts[i] > counts[top]) top = i;
+    *mode = lo + (top + 0.5) * dx;
+    int r = top + 1;
+    while (r < bins && !(counts[r] < 0.5 * counts[top])) ++r;
+    int l = top;
+    while (l > 0 && !(counts[l - 1] < 0.5 * counts[top])) --l;
+    *fwhm = r * dx - (l - 1) * dx;
+}
+
+// a path-or-number option as src/input/options.c:506-532 reads it
+static bool number(const char* s, double* v) {
+    if (!s || !*s) return false;
+    char* end;
+    *v = std::strtod(s, &end);
+    while (*end == ' ') ++end;
+    return *end == '\0';
+}
+
+static bool read_map(const char* opt, size_t w, size_t h, std::vector<float>& out, std::string& err) {
+    double v;
+    if (number(opt, &v)) {
+        out.assign(w * h, (float)v);
+        return true;
+    }
+    size_t mw, mh;
+    std::vector<double> d;
+    double pcs[4];
+    if (!read_image(opt, mw, mh, d, pcs, err)) return false;
+    if (mw != w || mh != h) {
+        err = std::string(opt) + ": wrong dimensions: " + std::to_string(mw) + " x " + std::to_string(mh) +
+              " (should be " + std::to_string(w) + " x " + std::to_string(h) + ")";
+        return false;
+    }
+    out.assign(d.begin(), d.end());
+    return true;
+}
+
+// src/lensed.c:406-549: image (with bscale), weights from a weight map or
+// from gain / (image + offset), extra weights, mask (weight 0 where the
+// mask is non-zero), PSF normalised to unit sum, and the checks of the
+// background mode and of the weights' signs
+int lensed_data_read(const lensed_data_options* o, lensed_data* d) {
+    if (!o || !d || !o->image) {
+        lsd::set_error("lensed_data_read: an image is required");
+        return -1;
+    }
+    std::memset(d, 0, sizeof *d);
+    std::string err;
+    std::vector<double> img;
+    size_t w, h;
+    if (!read_image(o->image, w, h, img, d->pcs, err)) {
+        lsd::set_error(err);
+        return -1;
+    }
+    std::vector<float> image(img.begin(), img.end()), weight;
+    if (o->bscale != 0)
+        for (auto& v : image) v = (float)(v * o->bscale);
+    if (o->weight) {
+        if (!read_map(o->weight, w, h, weight, err)) {
+            lsd::set_error(err);
+            return -1;
+        }
+    } else {
+        std::vector<float> gain;
+        if (!o->gain) {
+            lsd::set_error("lensed_data_read: either weight or gain is required");
+            return -1;
+        }
+        if (!read_map(o->gain, w, h, gain, err)) {
+            lsd::set_error(err);
+            return -1;
+        }
+        weight.resize(w * h);
+        for (size_t i = 0; i < w * h; ++i) weight[i] = (float)(gain[i] / (image[i] + o->offset));
+    }
+    if (o->xweight) {
+        std::vector<float> x;
+        if (!read_map(o->xweight, w, h, x, err)) {
+            lsd::set_error(err);
+            return -1;
+        }
+        for (size_t i = 0; i < w * h; ++i) weight[i] *= x[i];
+    }
+    if (o->mask) {
+        std::vector<float> m;
+        if (!read_map(o->mask, w, h, m, err)) {
+            lsd::set_error(err);
+            return -1;
+        }
+        for (size_t i = 0; i < w * h; ++i)
+            if (m[i] != 0) weight[i] = 0;
+    }
+    for (size_t i = 0; i < w * h; ++i) d->masked += weight[i] == 0;
+    std::vector<float> psf;
+    size_t pw = 0, ph = 0;
+    if (o->psf) {
+        std::vector<double> p;
+        double pp[4];
+        if (!read_image(o->psf, pw, ph, p, pp, err)) {
+            lsd::set_error(err);
+            return -1;
+        }
+        double norm = 0;
+        for (double v : p) norm += v;
+        psf.resize(p.size());
+        for (size_t i = 0; i < p.size(); ++i) psf[i] = (float)(p[i] / norm);
+    }
+    lensed_find_mode(w * h, image.data(), weight.data(), &d->mode, &d->fwhm);
+    if (std::fabs(d->mode) > 0.5 * d->fwhm) d->warnings |= LENSED_WARN_SKY_NOT_ZERO;
+    else if (o->offset == 0 && !o->weight) d->warnings |= LENSED_WARN_NO_OFFSET;
+    for (float v : weight)
+        if (!(v >= 0)) {
+            d->warnings |= LENSED_WARN_NEGATIVE_WEIGHT;
+            break;
+        }
+    d->width = w;
+    d->height = h;
+    d->image = (float*)std::malloc(w * h * sizeof(float));
+    d->weight = (float*)std::malloc(w * h * sizeof(float));
+    if (!d->image || !d->weight) {
+        lensed_data_free(d);
+        lsd::set_error("out of memory");
+        return -1;
+    }
+    std::memcpy(d->image, image.data(), w * h * sizeof(float));
+    std::memcpy(d->weight, weight.data(), w * h * sizeof(float));
+    if (!psf.empty()) {
+        d->psf = (float*)std::malloc(psf.size() * sizeof(float));
+        if (!d->psf) {
+            lensed_data_free(d);
+            lsd::set_error("out of memory");
+            return -1;
+        }
+        std::memcpy(d->psf, psf.data(), psf.size() * sizeof(float));
+        d->psfw = pw;
+        d->psfh = ph;
+    }
+    return 0;
+}
+
+void lensed_data_free(lensed_data* d) {
+    if (!d) return;
+    std::free(d->image);
+    std::free(d->weight);
+    std::free(d->psf);
+    d->image = d->weight = d->psf = nullptr;
+}
+
+}  // extern "C"

@@ -119,6 +119,11 @@ class LensedObject:
 class SystemSpec:
     """Objects, priors and data of one observed system."""
 
+    @classmethod
+    def from_observation(cls, objects, priors, obs, rule="g3k7"):
+        """A system from prepared data (lensed_amd.data.Observation)."""
+        return cls(objects, priors, obs.image, obs.weight, psf=obs.psf, pcs=obs.pcs, rule=rule)
+
     def __init__(self, objects, priors, image, weight, psf=None, pcs=(1.0, 1.0, 1.0, 1.0), rule="g3k7"):
         self.objects = [o if isinstance(o, LensedObject) else LensedObject(*o) for o in objects]
         byid = {o.id: o for o in self.objects}
@@ -241,6 +246,16 @@ class Context:
                                       planes["image"].ctypes.data_as(fp), planes["chi2"].ctypes.data_as(fp)),
                "render")
         return planes
+
+    def set_output(self, path):
+        """FITS file that every dumper() call writes the layers to (None: off)."""
+        _check(L.load().lensed_set_output(self.ptr, None if path is None else str(path).encode()), "set_output")
+
+    def write_layers(self, phys, path, s=0):
+        """IMG, RES, RAW, ERR, WHT and PVL of one parameter vector (object order)."""
+        p = np.ascontiguousarray(phys, dtype=np.float64)
+        _check(L.load().lensed_write_layers(self.ptr, s, p.ctypes.data_as(POINTER(c_double)), str(path).encode()),
+               "write_layers")
 
     def emulator_stats(self):
         """(fallbacks, candidates checked, largest held-out error) of the last emulated launch."""
