@@ -5,8 +5,9 @@ parameters and writes IMG (PSF-convolved model), RES (data - model), RAW
 (model before the PSF), ERR (quadrature error / model), WHT (weights) and
 PVL (erfc(sqrt(chi^2 / 2))) as named extensions.  IMG and RAW are checked
 against the FP64 oracle to the model-image bar (1e-4 relative where the
-model exceeds 1e-3 of its peak); the derived layers are checked against
-their definitions applied to those planes.
+model exceeds 1e-3 of its peak), ERR against the oracle's error plane to
+the bar the reference's own ERR layers meet (2e-5); the derived layers are
+checked against their definitions applied to those planes.
 """
 import ctypes
 from ctypes import POINTER, c_double, c_int
@@ -20,7 +21,7 @@ from lensed_amd import data as D
 from lensed_amd import workloads as W
 from lensed_amd.lensed import Context
 from oracle import oracle as O
-from test_gpu_parity import oracle_model, oracle_render_cb, rel_err
+from test_gpu_parity import PTS, WW64, oracle_model, oracle_render_cb, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -59,6 +60,6 @@ def test_dumper_writes_the_six_layers(tmp_path):
     assert np.array_equal(layers["WHT"], wl.weight)
     chi2 = wl.weight * (layers["IMG"].astype(np.float64) - wl.image) ** 2
     assert np.allclose(layers["PVL"], erfc(np.sqrt(0.5 * chi2)), atol=1e-5)
-    raw = layers["RAW"]
-    mask = np.abs(raw) > 1e-3 * np.abs(raw).max()
-    assert np.all(np.abs(layers["ERR"][mask]) < 1e-2)
+    val, err = O.render(wl.objects, truth, wl.shape, PTS, WW64, [1.0, 1.0, 1.0, 1.0])
+    mask = np.abs(val) > 1e-3 * np.abs(val).max()
+    assert np.max(np.abs(layers["ERR"][mask] - (err / val)[mask])) <= 2e-5
