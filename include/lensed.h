@@ -340,6 +340,36 @@ int lensed_set_output(lensed* l, const char* path);
 /* the same layers for one physical parameter vector (object order) */
 int lensed_write_layers(lensed* l, size_t s, const double* phys, const char* path);
 
+/* -------------------------------------------------------- configuration */
+
+/* src/input.c, src/input/ini.c, src/input/options.c -- Lensed's INI files
+ * ([options], [objects], [priors] with `wrap`/`image`, [labels]) and
+ * `--name=value` options, with the option table's types, defaults and
+ * conditional requirements; objects are loaded as they are named. */
+typedef struct lensed_config lensed_config;
+lensed_config* lensed_config_create(void);
+void lensed_config_free(lensed_config* c);
+/* read one INI file (relative paths are taken from its directory) */
+int lensed_config_read_ini(lensed_config* c, const char* path);
+/* set an option by (abbreviated) name, as an INI line or --name=value does */
+int lensed_config_set(lensed_config* c, const char* name, const char* value);
+/* one command-line argument "--name=value" (src/input.c:83-100) */
+int lensed_config_arg(lensed_config* c, const char* arg);
+/* src/input.c:214-239: required options, some objects, every prior given */
+int lensed_config_check(const lensed_config* c);
+/* an option's value as text (booleans "true"/"false"), NULL when unset */
+const char* lensed_config_get(const lensed_config* c, const char* name);
+/* the i-th entry of the option table; -1 past its end */
+int lensed_config_option(int i, const char** name, const char** type, const char** help, const char** defval);
+/* the objects in [objects] order, owned by the configuration */
+size_t lensed_config_nobjects(const lensed_config* c);
+object* lensed_config_objects(lensed_config* c);
+/* lensed_data_read() with the configuration's data options */
+int lensed_config_data(const lensed_config* c, lensed_data* data);
+/* src/lensed.c:1175-1227 -- <root>.paramnames (id, label) and <root>.ranges
+ * (id, prior lower and upper or N) in sampler order for system s */
+int lensed_write_paramfiles(const lensed* l, size_t s, const char* root);
+
 /* results recorded by dumper(): mean, sigma, ML, MAP per parameter in object
  * order (each may be NULL) and the log-evidence summary */
 int lensed_results(const lensed* l, double* mean, double* sigma, double* ml, double* map,

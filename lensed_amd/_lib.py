@@ -126,6 +126,19 @@ SIGNATURES = {
     "lensed_data_free": (None, [POINTER(Data)]),
     "lensed_set_output": (c_int, [c_void_p, c_char_p]),
     "lensed_write_layers": (c_int, [c_void_p, c_size_t, POINTER(c_double), c_char_p]),
+    "lensed_config_create": (c_void_p, []),
+    "lensed_config_free": (None, [c_void_p]),
+    "lensed_config_read_ini": (c_int, [c_void_p, c_char_p]),
+    "lensed_config_set": (c_int, [c_void_p, c_char_p, c_char_p]),
+    "lensed_config_arg": (c_int, [c_void_p, c_char_p]),
+    "lensed_config_check": (c_int, [c_void_p]),
+    "lensed_config_get": (c_char_p, [c_void_p, c_char_p]),
+    "lensed_config_option": (c_int, [c_int, POINTER(c_char_p), POINTER(c_char_p), POINTER(c_char_p),
+                                     POINTER(c_char_p)]),
+    "lensed_config_nobjects": (c_size_t, [c_void_p]),
+    "lensed_config_objects": (POINTER(Object), [c_void_p]),
+    "lensed_config_data": (c_int, [c_void_p, POINTER(Data)]),
+    "lensed_write_paramfiles": (c_int, [c_void_p, c_size_t, c_char_p]),
     "run": (None, [c_int, c_int, c_int, c_int, c_double, c_double, c_int, c_int, c_int, c_int, c_int, c_double,
                    c_char_p, c_int, POINTER(c_int), c_int, c_int, c_int, c_int, c_double, c_int, LOGLIKE_FN,
                    DUMPER_FN, c_void_p]),

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblensed_amd.so")
 ARCH = os.environ.get("LENSED_ARCH", "gfx950")
 
-HOST_SOURCES = ["capi.cpp", "context.cpp", "data.cpp", "nested.cpp", "prior.cpp", "program.cpp", "psf.cpp", "quadrature.cpp", "rtc.cpp"]
+HOST_SOURCES = ["capi.cpp", "config.cpp", "context.cpp", "data.cpp", "nested.cpp", "prior.cpp", "program.cpp", "psf.cpp", "quadrature.cpp", "rtc.cpp"]
 HIP_SOURCES = ["emulator.hip"]
 
 # packed FP32 instructions stay out of every HIP code object built here

@@ -19,6 +19,7 @@ void set_error(const std::string& msg);
 // one parameter of a system, with its own copy of the prior
 struct Param {
     std::string id;
+    std::string label;
     int type = 0;
     double lower = 0, upper = 0;
     bool bounded = false;

@@ -70,9 +70,19 @@ class Observation:
         o = L.DataOptions(image=opt(image), weight=opt(weight), gain=opt(gain), offset=float(offset),
                           bscale=float(bscale), xweight=opt(xweight), mask=opt(mask), psf=opt(psf))
         d = L.Data()
-        lib = L.load()
-        if lib.lensed_data_read(byref(o), byref(d)) != 0:
+        if L.load().lensed_data_read(byref(o), byref(d)) != 0:
             raise DataError(L.last_error())
+        self._take(d)
+
+    @classmethod
+    def _from_struct(cls, d):
+        obs = cls.__new__(cls)
+        obs._take(d)
+        return obs
+
+    def _take(self, d):
+        """Copy a lensed_data struct and free it."""
+        lib = L.load()
         try:
             shape = (d.height, d.width)
             self.image = np.ctypeslib.as_array(d.image, shape=shape).copy()

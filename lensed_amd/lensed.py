@@ -125,7 +125,7 @@ class SystemSpec:
         return cls(objects, priors, obs.image, obs.weight, psf=obs.psf, pcs=obs.pcs, rule=rule)
 
     def __init__(self, objects, priors, image, weight, psf=None, pcs=(1.0, 1.0, 1.0, 1.0), rule="g3k7"):
-        self.objects = [o if isinstance(o, LensedObject) else LensedObject(*o) for o in objects]
+        self.objects = [o if hasattr(o, "obj") else LensedObject(*o) for o in objects]
         byid = {o.id: o for o in self.objects}
         for key, text in priors.items():
             oid, pname = key.split(".", 1)
