@@ -57,13 +57,18 @@ def test_cli_run_on_reference_data(tmp_path, capsys, batch):
     assert "log-evidence" in out and "min chi²/dof" in out
     names = open(root + ".paramnames").read().splitlines()
     ranges = open(root + ".ranges").read().splitlines()
-    assert [n.split()[0] for n in names] == [k for k in ORDER if WIDTH[k]]
+    # sampled parameters first, fixed ones after them (MultiNest's derived parameters)
+    assert [n.split()[0] for n in names] == [k for k in ORDER if WIDTH[k]] + ["source.pa"]
+    assert ranges[-1].split()[1:] == ["0.0000", "0.0000"]
     assert names[2].split()[1] == "r_E"
     lo, hi = map(float, ranges[2].split()[1:])
     assert (lo, hi) == (19.0, 21.0)
     layers = {n: D.read_fits(f"{root}.fits[{n}]")[0] for n in D.LAYERS}
     ref, _ = D.read_fits(os.path.join(HERE, "golden", "reference_sie.fits"))
-    assert np.max(np.abs(layers["RES"])) <= 1e-3 * np.max(ref)   # the ML model reproduces the reference render
+    # the best sample of 100 live points lies within a few sigma of the truth, so
+    # its model reproduces the reference render to well under 1 % of the peak
+    assert np.max(np.abs(layers["RES"])) <= 1e-2 * np.max(ref)
+    assert np.array_equal(layers["RES"], ref - layers["IMG"])
     assert np.all(layers["WHT"] == 1000)
     post = np.loadtxt(root + ".txt")
     assert post.shape[1] == 2 + 12
