@@ -462,13 +462,18 @@ __global__ __launch_bounds__(256) void k_emu_assemble(const lensed_desc* __restr
     int s = find_sys_t<0>(sys, nsys, item);
     const lensed_desc& d = sys[s];
     int local = item - d.item_t;
-    int cand = local / (2 * LENSED_EPL_NT);
-    int rem = local - cand * 2 * LENSED_EPL_NT;
-    int half = rem / LENSED_EPL_NT, i = rem - half * LENSED_EPL_NT;
+    int cand = local / (2 * LENSED_EPL_ROWS);
+    int rem = local - cand * 2 * LENSED_EPL_ROWS;
+    int half = rem / LENSED_EPL_ROWS, i = rem - half * LENSED_EPL_ROWS;
     long long cb = d.table_off / LENSED_EPL_TABLE_F4 + cand;
     float4 a = nodes[cb * kNodes + half * (LENSED_EPL_NT + 1) + i];
-    float4 b = nodes[cb * kNodes + half * (LENSED_EPL_NT + 1) + i + 1];
     float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4 + 2 * rem;
+    if (i == LENSED_EPL_NT) {  // the end point tau = 1 as a constant row
+        out[0] = float4(a.x, 0.0f, 0.0f, 0.0f);
+        out[1] = float4(a.y, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    float4 b = nodes[cb * kNodes + half * (LENSED_EPL_NT + 1) + i + 1];
     out[0] = lensed_hermite(a.x, a.z, b.x, b.z);
     out[1] = lensed_hermite(a.y, a.w, b.y, b.w);
 }
@@ -495,7 +500,7 @@ __global__ __launch_bounds__(256) void k_emu_check(const lensed_desc* __restrict
         int half = m / LENSED_EPL_NT, i = m - half * LENSED_EPL_NT;
         double ox, oy, dx, dy;
         lensed_epl_node(q, t, half, (i + 0.5) / LENSED_EPL_NT, ox, oy, dx, dy);
-        float4 cx = tab[2 * m], cy = tab[2 * m + 1];
+        float4 cx = tab[2 * (half * LENSED_EPL_ROWS + i)], cy = tab[2 * (half * LENSED_EPL_ROWS + i) + 1];
         float f = 0.5f;
         float tx = cx.x + f * (cx.y + f * (cx.z + f * cx.w));
         float ty = cy.x + f * (cy.y + f * (cy.z + f * cy.w));

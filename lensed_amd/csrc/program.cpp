@@ -232,8 +232,29 @@ static std::string data_ptr(const std::string& m, size_t off, const char* base) 
 
 std::string compute_function(const std::vector<ModelObject>& objs, int k) {
     std::ostringstream o;
-    o << "static inline float compute_" << k << "(uint* data, float2 x, const float4* tab)\n{\n"
-      << "    (void)tab;\n"
+    // a tabulated EPL lens (at most one per model) starts from the pair
+    // (F (x - c), S (x - c)) that the render kernel forms incrementally
+    o << "static inline float4 epl_pre_" << k << "(uint* data, float2 d, bool pixel)\n{\n"
+      << "    (void)data; (void)d; (void)pixel;\n";
+    {
+        size_t dd = 0;
+        bool any = false;
+        for (const auto& ob : objs) {
+            if (ob.table && !any) {
+                std::string m = mangle(ob.name);
+                o << "    const struct data_" << m << "* s = " << data_ptr(m, dd, "data") << ";\n"
+                  << "    return lensed_epl_pre" << (ob.name == "epl_plus_shear" ? "_shear" : "")
+                  << "(s, pixel ? d - s->c : d);\n";
+                any = true;
+            }
+            dd += ob.size;
+        }
+        if (!any) o << "    return float4(0.0f, 0.0f, 0.0f, 0.0f);\n";
+    }
+    o << "}\n";
+    o << "static inline float compute_" << k
+      << "(uint* data, float2 x, const float4* tab, float4 pre = float4(0.0f, 0.0f, 0.0f, 0.0f))\n{\n"
+      << "    (void)tab; (void)pre;\n"
       << "    float2 y = x;\n"
       << "    float f = 0;\n";
     bool plane_open = false;
@@ -256,7 +277,7 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
             }
             if (ob.table)
                 o << "        a += lensed_epl_table_deflection" << (ob.name == "epl_plus_shear" ? "_shear" : "")
-                  << "(" << data_ptr(m, d, "data") << ", y, tab);\n";
+                  << "_pre(" << data_ptr(m, d, "data") << ", pre, tab);\n";
             else
                 o << "        a += deflection_" << m << "(" << data_ptr(m, d, "data") << ", y);\n";
         } else if (ob.type == OBJ_SOURCE) {

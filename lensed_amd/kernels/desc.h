@@ -13,7 +13,8 @@
 #define LENSED_CONV_BS (LENSED_CONV_TX * LENSED_CONV_TY)
 #define LENSED_PSF_MAX 63     // largest PSF side the conv tile is sized for
 #define LENSED_EPL_NT 128     // EPL angular table: intervals per octant
-#define LENSED_EPL_TABLE_F4 (2 * LENSED_EPL_NT * 2)  // float4 per candidate (2 octants x 2 components)
+#define LENSED_EPL_ROWS (LENSED_EPL_NT + 1)  // rows per octant: the intervals and the end point tau = 1
+#define LENSED_EPL_TABLE_F4 (2 * LENSED_EPL_ROWS * 2)  // float4 per candidate (2 octants x 2 components)
 
 struct lensed_desc {
     int width, height, npix, model;  // image size, pixels, model index

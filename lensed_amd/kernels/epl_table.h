@@ -9,53 +9,89 @@
 // max(|u_x|, |u_y|) in [0, 1]: half 0 is psi = atan(tau), half 1 is
 // psi = pi/2 - atan(tau).  Each half has LENSED_EPL_NT intervals; per
 // interval the table holds the cubic Hermite polynomial in the local
-// coordinate f in [0, 1] for each component, as two float4 (a0, a1, a2, a3):
-//     O(f) = a0 + f (a1 + f (a2 + f a3)).
+// coordinate f in [0, 1) for each component, as two float4 (a0, a1, a2, a3):
+//     O(f) = a0 + f (a1 + f (a2 + f a3)),
+// and one more row per half holds the value at tau = 1 (a1 = a2 = a3 = 0),
+// so the row index floor(tau NT) needs no clamp.
 // The table of a candidate is built from the series (or by the emulator)
 // and staged in LDS by the render kernel.
+//
+// The lens-frame position u = F (x - c) and the shear term S (x - c) are
+// linear in the ray position x = p + q_n (pixel centre plus quadrature
+// node), so the render kernel forms them as F (p - c) + F q_n with F q_n
+// computed once per node and candidate (lensed_epl_pre*); the deflection
+// then starts from the pair (u, S (x - c)).
 #pragma once
 
-// LENSED_EPL_NT (intervals per octant) and LENSED_EPL_TABLE_F4 (float4 per
-// candidate) come from desc.h.
+// LENSED_EPL_NT (intervals per octant), LENSED_EPL_ROWS and
+// LENSED_EPL_TABLE_F4 (float4 per candidate) come from desc.h.
 
 #ifdef LCL_GPU
 #pragma clang force_cuda_host_device begin
 #endif
 namespace lcl {
 
-inline float2 lensed_epl_omega_tab(float2 u, float uu, const float4* tab) {
+inline float lensed_fract(float s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fractf(s);
+#else
+    return s - floorf(s);
+#endif
+}
+
+inline float2 lensed_epl_omega_tab(float2 u, const float4* tab) {
     float au = fabs(u.x), av = fabs(u.y);
-    float lo = fmin(au, av), hi = fmax(au, av);
-    float tau = uu > 0 ? lo / hi : 0.0f;
-    float s = tau * (float)LENSED_EPL_NT;
-    int i = min((int)s, LENSED_EPL_NT - 1);
-    float f = s - (float)i;
-    int half = av > au ? 1 : 0;
-    const float4 cx = tab[2 * (half * LENSED_EPL_NT + i)];
-    const float4 cy = tab[2 * (half * LENSED_EPL_NT + i) + 1];
+    // the floor keeps tau = 0 (not 0/0) at u = 0
+    float lo = fmin(au, av), hi = fmax(fmax(au, av), 1e-30f);
+    float s = (lo / hi) * (float)LENSED_EPL_NT;
+    int i = (int)s;
+    float f = lensed_fract(s);
+    const float4* row = (av > au ? tab + 2 * LENSED_EPL_ROWS : tab) + 2 * i;
+    const float4 cx = row[0];
+    const float4 cy = row[1];
     float ox = cx.x + f * (cx.y + f * (cx.z + f * cx.w));
     float oy = cy.x + f * (cy.y + f * (cy.z + f * cy.w));
     return float2(copysign(ox, u.x), copysign(oy, u.y));
 }
 
-// the epl / epl_plus_shear deflection with the tabulated angular factor;
-// D is the object's data struct (fields c, fwd, bwd, [shear], hexp, norm)
+// (F d, 0, 0) and (F d, S d): the lens-frame position and the shear term of
+// an offset d; D is the object's data struct (fields c, fwd, bwd, [shear],
+// hexp, norm)
 template <class D>
-inline float2 lensed_epl_table_deflection(const D* self, float2 x, const float4* tab) {
-    float2 d = x - self->c;
+inline float4 lensed_epl_pre(const D* self, float2 d) {
     float2 u = mv22(self->fwd, d);
+    return float4(u.x, u.y, 0.0f, 0.0f);
+}
+
+template <class D>
+inline float4 lensed_epl_pre_shear(const D* self, float2 d) {
+    float2 u = mv22(self->fwd, d), g = mv22(self->shear, d);
+    return float4(u.x, u.y, g.x, g.y);
+}
+
+// the deflection from the pair; pre = (u, S (x - c))
+template <class D>
+inline float2 lensed_epl_table_deflection_pre(const D* self, float4 pre, const float4* tab) {
+    float2 u = float2(pre.x, pre.y);
     float uu = dot(u, u);
-    float2 om = lensed_epl_omega_tab(u, uu, tab);
+    float2 om = lensed_epl_omega_tab(u, tab);
     return mv22(self->bwd, (self->norm * powr(uu, self->hexp)) * om);
 }
 
 template <class D>
+inline float2 lensed_epl_table_deflection_shear_pre(const D* self, float4 pre, const float4* tab) {
+    return lensed_epl_table_deflection_pre(self, pre, tab) + float2(pre.z, pre.w);
+}
+
+// the same from a position
+template <class D>
+inline float2 lensed_epl_table_deflection(const D* self, float2 x, const float4* tab) {
+    return lensed_epl_table_deflection_pre(self, lensed_epl_pre(self, x - self->c), tab);
+}
+
+template <class D>
 inline float2 lensed_epl_table_deflection_shear(const D* self, float2 x, const float4* tab) {
-    float2 d = x - self->c;
-    float2 u = mv22(self->fwd, d);
-    float uu = dot(u, u);
-    float2 om = lensed_epl_omega_tab(u, uu, tab);
-    return mv22(self->bwd, (self->norm * powr(uu, self->hexp)) * om) + mv22(self->shear, d);
+    return lensed_epl_table_deflection_shear_pre(self, lensed_epl_pre_shear(self, x - self->c), tab);
 }
 
 }  // namespace lcl

@@ -77,12 +77,20 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
 #pragma unroll
     for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = i < d.dwords ? __builtin_amdgcn_readfirstlane(gb[i]) : 0u;
+    const float4* q = quad + d.quad_off;
     const lcl::float4* tab = nullptr;
-    if (d.table_mode) {  // the candidate's EPL angular table, staged in LDS
+    const lcl::float4* npre = nullptr;
+    if (d.table_mode) {
+        // the candidate's EPL angular table, then the lens-frame offsets of
+        // the quadrature nodes (F q_n, S q_n), staged in LDS
         const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
         for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
+        lcl::float4* pre_lds = (lcl::float4*)(tab_lds + LENSED_EPL_TABLE_F4);
+        for (int n = threadIdx.x; n < d.nq; n += LENSED_RENDER_BS)
+            pre_lds[n] = lcl::epl_pre_model<K>(blk, lcl::float2(q[n].x, q[n].y), false);
         __syncthreads();
         tab = (const lcl::float4*)tab_lds;
+        npre = pre_lds;
     }
     int k = tile * LENSED_RENDER_BS + (int)threadIdx.x;
     if (k < d.npix) {
@@ -91,13 +99,27 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
         // pixel centre as kernel/lensed.cl:25, then one ray per node
         const float cx = d.pcs[0] + d.pcs[2] * (float)px;
         const float cy = d.pcs[1] + d.pcs[3] * (float)py;
-        const float4* q = quad + d.quad_off;
         float f = 0.0f, e = 0.0f;
-        for (int n = 0; n < d.nq; ++n) {
-            const float4 node = q[n];
-            float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y), tab);
-            f += node.z * v;
-            if (ERR) e += node.w * v;
+        if (d.table_mode) {
+            // F (p - c) + F q_n: the lens-frame pair of each ray from the
+            // pixel's and the node's parts
+            const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
+            for (int n = 0; n < d.nq; ++n) {
+                const float4 node = q[n];
+                const lcl::float4 np = npre[n];
+                const lcl::float4 pre(base.x + np.x, base.y + np.y, base.z + np.z, base.w + np.w);
+                float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y), tab, pre);
+                f += node.z * v;
+                if (ERR) e += node.w * v;
+            }
+        } else {
+            for (int n = 0; n < d.nq; ++n) {
+                const float4 node = q[n];
+                float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y), tab,
+                                                lcl::float4(0.0f, 0.0f, 0.0f, 0.0f));
+                f += node.z * v;
+                if (ERR) e += node.w * v;
+            }
         }
         long long o = d.model_off + (long long)cand * d.npix + k;
         model[o] = f;
@@ -132,10 +154,10 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
 }
 
 // -------------------------------------------------------------- EPL tables
-// One thread per (candidate, octant, interval): the angular factor Omega and
+// One thread per (candidate, octant, row): the angular factor Omega and
 // dOmega/dtau of the tenth-order series at both ends of the interval, in
 // FP64 from the candidate's (q, t), turned into cubic Hermite coefficients
-// (epl_series.h).  With `only` set, just the flagged candidates (the
+// (epl_series.h); the last row of an octant is the constant value at tau = 1.  With `only` set, just the flagged candidates (the
 // emulator's fallback).
 extern "C" __global__ __launch_bounds__(256) void lensed_epl_tables(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
@@ -146,16 +168,22 @@ extern "C" __global__ __launch_bounds__(256) void lensed_epl_tables(
     int s = lensed_find_sys<4>(sys, nsys, item);
     const lensed_desc& d = sys[s];
     int local = item - d.item_t;
-    int cand = local / (2 * LENSED_EPL_NT);
-    int rem = local - cand * 2 * LENSED_EPL_NT;
-    int half = rem / LENSED_EPL_NT, i = rem - half * LENSED_EPL_NT;
+    int cand = local / (2 * LENSED_EPL_ROWS);
+    int rem = local - cand * 2 * LENSED_EPL_ROWS;
+    int half = rem / LENSED_EPL_ROWS, i = rem - half * LENSED_EPL_ROWS;
     if (only && !only[d.table_off / LENSED_EPL_TABLE_F4 + cand]) return;
     const float* p = params + d.param_off + (long long)cand * d.npar;
     double q = p[d.epl_q], t = p[d.epl_t];
     double x0, y0, dx0, dy0, x1, y1, dx1, dy1;
+    float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4 + 2 * rem;
+    if (i == LENSED_EPL_NT) {  // the end point tau = 1 as a constant row
+        lensed_epl_node(q, t, half, 1.0, x1, y1, dx1, dy1);
+        out[0] = float4((float)x1, 0.0f, 0.0f, 0.0f);
+        out[1] = float4((float)y1, 0.0f, 0.0f, 0.0f);
+        return;
+    }
     lensed_epl_node(q, t, half, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
     lensed_epl_node(q, t, half, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
-    float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4 + 2 * rem;
     out[0] = lensed_hermite(x0, dx0, x1, dx1);
     out[1] = lensed_hermite(y0, dy0, y1, dy1);
 }

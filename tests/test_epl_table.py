@@ -74,3 +74,128 @@ def test_series_derivative_is_consistent():
     b = O.epl_omega(q, t, psi - h)
     assert np.allclose((a - b)[:, 0] / (2 * h), dx, atol=1e-7)
     assert np.allclose((a - b)[:, 1] / (2 * h), dy, atol=1e-7)
+
+
+# ------------------------------------------------------------------------
+# the product's table code (kernels/epl_table.h inside the generated model
+# source) compiled for the host: lookups, the tau = 1 end row, u = 0, and the
+# deflection from the pair (F (p - c) + F q, S (p - c) + S q) as the render
+# kernel forms it, against the FP64 oracle's deflection
+
+import ctypes  # noqa: E402
+import hashlib  # noqa: E402
+import os  # noqa: E402
+import subprocess  # noqa: E402
+import tempfile  # noqa: E402
+from ctypes import POINTER, byref, c_double, c_float, c_int  # noqa: E402
+
+from lensed_amd import _lib as L  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+HARNESS = r'''
+#define __device__
+namespace hostseries {
+using float4 = lcl::float4;
+using ::sincos; using ::atan;
+inline float4 make_float4(float a, float b, float c, float d) { return float4(a, b, c, d); }
+@SERIES@
+}
+static void build_table(double q, double t, lcl::float4* tab) {
+    for (int half = 0; half < 2; ++half)
+        for (int i = 0; i <= LENSED_EPL_NT; ++i) {
+            double x0, y0, dx0, dy0, x1, y1, dx1, dy1;
+            lcl::float4* out = tab + 2 * (half * LENSED_EPL_ROWS + i);
+            if (i == LENSED_EPL_NT) {
+                hostseries::lensed_epl_node(q, t, half, 1.0, x1, y1, dx1, dy1);
+                out[0] = lcl::float4((float)x1, 0.f, 0.f, 0.f);
+                out[1] = lcl::float4((float)y1, 0.f, 0.f, 0.f);
+                continue;
+            }
+            hostseries::lensed_epl_node(q, t, half, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
+            hostseries::lensed_epl_node(q, t, half, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
+            out[0] = hostseries::lensed_hermite(x0, dx0, x1, dx1);
+            out[1] = hostseries::lensed_hermite(y0, dy0, y1, dy1);
+        }
+}
+extern "C" void host_omega(double q, double t, const float* u, int n, float* out) {
+    static lcl::float4 tab[LENSED_EPL_TABLE_F4];
+    build_table(q, t, tab);
+    for (int k = 0; k < n; ++k) {
+        lcl::float2 o = lcl::lensed_epl_omega_tab(lcl::float2(u[2 * k], u[2 * k + 1]), tab);
+        out[2 * k] = o.x; out[2 * k + 1] = o.y;
+    }
+}
+// deflection of object 0 (epl_plus_shear) at pixel p plus node q, from the split pair
+extern "C" void host_deflect(const float* params, const float* p, const float* qn, int n, float* out) {
+    static lcl::uint data[4096];
+    lcl::set_params_0(data, params);
+    const auto* s = (const lcl::data_epl_plus_shear*)data;
+    static lcl::float4 tab[LENSED_EPL_TABLE_F4];
+    build_table(params[4], params[3], tab);
+    for (int k = 0; k < n; ++k) {
+        lcl::float2 px(p[2 * k], p[2 * k + 1]), qq(qn[2 * k], qn[2 * k + 1]);
+        lcl::float4 b = lcl::lensed_epl_pre_shear(s, px - s->c), d = lcl::lensed_epl_pre_shear(s, qq);
+        lcl::float4 pre(b.x + d.x, b.y + d.y, b.z + d.z, b.w + d.w);
+        lcl::float2 a = lcl::lensed_epl_table_deflection_shear_pre(s, pre, tab);
+        out[2 * k] = a.x; out[2 * k + 1] = a.y;
+    }
+}
+'''
+
+
+def _host_lib():
+    lib = L.load()
+    arr = (L.Object * 2)()
+    for i, (oid, name) in enumerate([("lens", "epl_plus_shear"), ("src", "gauss")]):
+        assert lib.lensed_object_load(byref(arr[i]), oid.encode(), name.encode()) == 0
+    src = L.take_string(lib.lensed_model_source(2, arr))
+    series = open(os.path.join(ROOT, "lensed_amd", "kernels", "epl_series.h")).read().replace("#pragma once", "")
+    src += HARNESS.replace("@SERIES@", series)
+    d = tempfile.mkdtemp(prefix="lensed_tab_")
+    cpp, so = os.path.join(d, "t.cpp"), os.path.join(d, "t.so")
+    open(cpp, "w").write(src)
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-w", "-o", so, cpp])
+    h = ctypes.CDLL(so)
+    fp = POINTER(c_float)
+    h.host_omega.argtypes = [c_double, c_double, fp, c_int, fp]
+    h.host_deflect.argtypes = [fp, fp, fp, c_int, fp]
+    return h
+
+
+def test_product_table_lookup_on_the_host():
+    h = _host_lib()
+    rng = np.random.default_rng(4)
+    fp = POINTER(c_float)
+    for q, t in [(0.3, 0.5), (0.7, 1.1), (0.95, 1.45), (1.0, 1.0)]:
+        phi = rng.uniform(-np.pi, np.pi, 3000)
+        r = rng.uniform(0.05, 5.0, 3000)
+        u = np.stack([r * np.cos(phi), r * np.sin(phi)], 1)
+        # exact diagonals (tau = 1, the end row) and axes (tau = 0)
+        u[:8] = [[1, 1], [-2, 2], [3, -3], [-0.5, -0.5], [1, 0], [0, -2], [-3, 0], [0, 4]]
+        phi[:8] = np.arctan2(u[:8, 1], u[:8, 0])
+        u = u.astype(np.float32)
+        out = np.zeros_like(u)
+        h.host_omega(q, t, u.ctypes.data_as(fp), len(u), out.ctypes.data_as(fp))
+        exact = O.epl_omega(q, t, phi)
+        assert np.max(np.abs(out - exact)) < 2e-6, (q, t, np.max(np.abs(out - exact)))
+    # u = 0 gives the tau = 0 row, not a NaN
+    z = np.zeros((1, 2), np.float32)
+    h.host_omega(0.6, 1.2, z.ctypes.data_as(fp), 1, z.ctypes.data_as(fp))
+    assert np.all(np.isfinite(z))
+
+
+def test_split_pair_deflection_matches_oracle():
+    h = _host_lib()
+    rng = np.random.default_rng(5)
+    fp = POINTER(c_float)
+    objs = [("lens", "epl_plus_shear"), ("src", "gauss")]
+    params = np.array([50.3, 49.7, 12.0, 1.2, 0.65, 35.0, 0.03, -0.02, 50.0, 50.0, 2.0, 0.0, 0.8, 10.0])
+    p = rng.uniform(30, 70, (2000, 2)).astype(np.float32)
+    qn = rng.uniform(-0.5, 0.5, (2000, 2)).astype(np.float32)
+    out = np.zeros_like(p)
+    h.host_deflect(params.astype(np.float32).ctypes.data_as(fp), p.ctypes.data_as(fp), qn.ctypes.data_as(fp),
+                   len(p), out.ctypes.data_as(fp))
+    exact = O.deflect(objs, params, 0, p.astype(np.float64) + qn.astype(np.float64))
+    scale = np.max(np.abs(exact))
+    assert np.max(np.abs(out - exact)) < 2e-5 * scale, np.max(np.abs(out - exact)) / scale
