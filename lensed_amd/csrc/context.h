@@ -65,6 +65,7 @@ struct Layout {
     int n_nodes = 0, n_tcands = 0;  // emulator: table nodes and candidates with a table
     size_t lds_conv = 0, lds_render = 0;
     bool conv_small = false;        // short tiles: launch too small to fill the GPU
+    int grid_conv = 0, grid_render = 0;  // resident blocks for this launch's LDS use
     lensed_desc* d_desc = nullptr;
     hipGraphExec_t graph = nullptr;
 };

@@ -231,18 +231,41 @@ __device__ inline double lensed_wave_csum(float s, float c) {
 }
 
 // Stage the model tile plus halo (clamped to the image) in LDS; row stride
-// LENSED_TILE_STRIDE(PW) keeps rows 16-byte aligned for vector reads.
+// LENSED_TILE_STRIDE(PW) keeps rows 16-byte aligned for vector reads.  A
+// thread owns columns lx and lx + 64 (the halo is narrower than 64) of
+// every fourth row; the loads of eight rows are issued together before any
+// is stored, so their latencies overlap.
 template <int RY>
 __device__ inline void lensed_conv_load_tile(const lensed_desc& d, const float* __restrict__ m, float* tile,
                                              int tx0, int ty0, int lx, int ly, int PW, int PH)
 {
+    constexpr int TX = LENSED_CONV_TX, TY = LENSED_CONV_TY, NB = 8;
     const int W = d.width, H = d.height;
-    const int cw = LENSED_CONV_TX + PW - 1, ch = LENSED_CONV_TY * RY + PH - 1;
+    const int cw = TX + PW - 1, ch = TY * RY + PH - 1;
     const int stride = LENSED_TILE_STRIDE(PW);
     const int ox = tx0 - PW / 2, oy = ty0 - PH / 2;
-    for (int r = ly; r < ch; r += LENSED_CONV_TY) {
-        const float* src = m + min(max(oy + r, 0), H - 1) * W;
-        for (int c = lx; c < cw; c += LENSED_CONV_TX) tile[r * stride + c] = src[min(max(ox + c, 0), W - 1)];
+    const int c0 = min(max(ox + lx, 0), W - 1), c1 = min(max(ox + lx + TX, 0), W - 1);
+    const bool second = lx + TX < cw;
+    for (int r0 = ly; r0 < ch; r0 += NB * TY) {
+        float v0[NB], v1[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int r = r0 + k * TY;
+            v0[k] = v1[k] = 0.0f;
+            if (r < ch) {
+                const float* src = m + min(max(oy + r, 0), H - 1) * W;
+                v0[k] = src[c0];
+                if (second) v1[k] = src[c1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int r = r0 + k * TY;
+            if (r < ch) {
+                tile[r * stride + lx] = v0[k];
+                if (second) tile[r * stride + lx + TX] = v1[k];
+            }
+        }
     }
 }
 
@@ -372,9 +395,24 @@ __device__ inline void lensed_conv_item(const lensed_desc& d, int cand, int t,
     float acc[RY];
 #pragma unroll
     for (int r = 0; r < RY; ++r) acc[r] = 0.0f;
+    // data and weights of this thread's pixels, requested once the tile is
+    // staged, so that their loads overlap the convolution
+    float dat[RY], wt[RY];
+    auto fetch = [&]() {
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int gy = ty0 + ly * RY + r;
+            dat[r] = wt[r] = 0.0f;
+            if (gx < W && gy < H) {
+                dat[r] = img[gy * W + gx];
+                wt[r] = wgt[gy * W + gx];
+            }
+        }
+    };
     if (d.psfw > 0) {
         lensed_conv_load_tile<RY>(d, m, tile, tx0, ty0, lx, ly, d.psfw, d.psfh);
         __syncthreads();
+        fetch();
         bool done = false;
 #define LENSED_PSF_CASE(PW_, PH_, SEP_)                                                  \
         if (!done && d.psfw == PW_ && d.psfh == PH_ && (d.sep_rank > 0) == SEP_) {       \
@@ -385,6 +423,7 @@ __device__ inline void lensed_conv_item(const lensed_desc& d, int cand, int t,
 #undef LENSED_PSF_CASE
         if (!done) lensed_conv_generic<RY>(d, psf, tile, lx, ly, acc);
     } else {
+        fetch();
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             int gy = ty0 + ly * RY + r;
@@ -398,8 +437,8 @@ __device__ inline void lensed_conv_item(const lensed_desc& d, int cand, int t,
         chi[r] = 0.0f;
         if (gx < W && gy < H) {
             int kk = gy * W + gx;
-            float dm = acc[r] - img[kk];
-            chi[r] = wgt[kk] * dm * dm;
+            float dm = acc[r] - dat[r];
+            chi[r] = wt[r] * dm * dm;
             if (conv_out) conv_out[d.model_off + (long long)cand * d.npix + kk] = acc[r];
             if (chi2_out) chi2_out[d.model_off + (long long)cand * d.npix + kk] = chi[r];
         }
