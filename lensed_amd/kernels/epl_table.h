@@ -56,7 +56,7 @@ inline float2 lensed_epl_omega_tab(float2 u, const float4* tab) {
 
 // (F d, 0, 0) and (F d, S d): the lens-frame position and the shear term of
 // an offset d; D is the object's data struct (fields c, fwd, bwd, [shear],
-// hexp, norm)
+// hexp; bwd carries the normalisation)
 template <class D>
 inline float4 lensed_epl_pre(const D* self, float2 d) {
     float2 u = mv22(self->fwd, d);
@@ -75,7 +75,7 @@ inline float2 lensed_epl_table_deflection_pre(const D* self, float4 pre, const f
     float2 u = float2(pre.x, pre.y);
     float uu = dot(u, u);
     float2 om = lensed_epl_omega_tab(u, tab);
-    return mv22(self->bwd, (self->norm * powr(uu, self->hexp)) * om);
+    return mv22(self->bwd, powr(uu, self->hexp) * om);
 }
 
 template <class D>

@@ -23,10 +23,9 @@ data
 {
     float2 c;       // centre
     mat22 fwd;      // image frame -> elliptical frame, scaled by 1/(r sqrt(q))
-    mat22 bwd;      // elliptical frame -> image frame (rotation only)
+    mat22 bwd;      // elliptical frame -> image frame: rotation times 2 r sqrt(q)/(1 + q)
     float coef[10]; // C_k = prod_{j<=k} -f (2j - T)/(2j + T), k = 1..10
     float hexp;     // (1 - t)/2, exponent applied to |u|^2
-    float norm;     // 2 r sqrt(q)/(1 + q)
 };
 
 static float2 deflection(local data* this, float2 x)
@@ -41,7 +40,7 @@ static float2 deflection(local data* this, float2 x)
         p = (float2)(this->coef[k] + z.x*p.x - z.y*p.y, z.y*p.x + z.x*p.y);
     p = (float2)(1 + z.x*p.x - z.y*p.y, z.y*p.x + z.x*p.y);
     float2 sum = (float2)(e.x*p.x - e.y*p.y, e.y*p.x + e.x*p.y);
-    return mv22(this->bwd, (this->norm*powr(uu, this->hexp))*sum);
+    return mv22(this->bwd, powr(uu, this->hexp)*sum);
 }
 
 static void set(local data* this, float x, float y, float r, float t, float q, float pa)
@@ -52,7 +51,7 @@ static void set(local data* this, float x, float y, float r, float t, float q, f
     float f = (1 - q)/(1 + q);
     this->c = (float2)(x, y);
     this->fwd = (1/r/sqrt(q))*(mat22)(q*cs, q*sn, -sn, cs);
-    this->bwd = (mat22)(cs, -sn, sn, cs);
+    this->bwd = (2*r*sqrt(q)/(1 + q))*(mat22)(cs, -sn, sn, cs);
     float c = 1;
     for(int k = 1; k <= 10; ++k)
     {
@@ -60,5 +59,4 @@ static void set(local data* this, float x, float y, float r, float t, float q, f
         this->coef[k-1] = c;
     }
     this->hexp = 0.5f*(1 - t);
-    this->norm = 2*r*sqrt(q)/(1 + q);
 }

@@ -14,15 +14,15 @@ params
 data
 {
     float2 c;   // centre
-    mat22 tr;   // rotation and axis scaling
-    float k2;   // -log2(e)/(2 sigma^2)
-    float peak; // central surface brightness L/(2 pi sigma^2 q)
+    mat22 tr;   // rotation and axis scaling, times sqrt(log2(e)/2)/sigma
+    float lpk;  // log2 of the central surface brightness L/(2 pi sigma^2 q)
 };
 
+// peak exp(-R^2/(2 sigma^2)) as exp2(log2(peak) - |tr (x - c)|^2)
 static float brightness(local data* this, float2 x)
 {
     float2 u = mv22(this->tr, x - this->c);
-    return this->peak*exp2(this->k2*dot(u, u));
+    return exp2(fma(-u.x, u.x, fma(-u.y, u.y, this->lpk)));
 }
 
 static void set(local data* this, float x, float y, float sigma, float mag, float q, float pa)
@@ -30,7 +30,6 @@ static void set(local data* this, float x, float y, float sigma, float mag, floa
     float cs = cos(pa*DEG2RAD);
     float sn = sin(pa*DEG2RAD);
     this->c = (float2)(x, y);
-    this->tr = (mat22)(q*cs, q*sn, -sn, cs);
-    this->k2 = -0.72134752044448170368f/(sigma*sigma);
-    this->peak = exp(-0.4f*mag*LOG_10)*0.5f/PI/(sigma*sigma)/q;
+    this->tr = (0.84932180028801904272f/sigma)*(mat22)(q*cs, q*sn, -sn, cs);
+    this->lpk = -0.4f*mag*3.32192809488736234787f - log2(2*PI*sigma*sigma*q);
 }
