@@ -261,7 +261,7 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
     int trigger = 0;
     size_t d = 0;
     auto close_plane = [&]() {
-        o << "        y -= dot(a, a) < HUGE_VALF ? a : float2(1E10f, 1E10f);\n    }\n";
+        o << "        y -= lensed_deflection_ok(a) ? a : float2(1E10f, 1E10f);\n    }\n";
         plane_open = false;
     };
     for (const auto& ob : objs) {

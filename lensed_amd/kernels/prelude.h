@@ -206,6 +206,14 @@ inline float2 max(float2 a, float b) { return float2(max(a.x, b), max(a.y, b)); 
 inline float dot(float2 a, float2 b) { return a.x * b.x + a.y * b.y; }
 inline float dot(float4 a, float4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
 inline float dot(float a, float b) { return a * b; }
+
+// The ray-shooting guard of the generated compute(): a deflection is used
+// when it is finite and small enough, else the ray goes to (1e10, 1e10).
+// The test |a_x| + |a_y| < 2^64 rejects exactly the non-finite deflections
+// that dot(a, a) < HUGE_VALF rejects (any inf or NaN component); the two
+// differ only for finite |a| between 1.3e19 and 1.8e19 pixels, where either
+// ray ends far outside every source.
+inline bool lensed_deflection_ok(float2 a) { return fabs(a.x) + fabs(a.y) < 1.8446744e19f; }
 inline float length(float2 a) { return ::sqrtf(a.x * a.x + a.y * a.y); }
 inline float length(float4 a) { return ::sqrtf(dot(a, a)); }
 inline float fast_length(float2 a) { return length(a); }
