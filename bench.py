@@ -39,9 +39,9 @@ PEAK_FP32 = 157.3  # TFLOP/s, MI355X published FP32 (packed / matrix) peak
 PEAK_FP32_NONPACKED = 78.6
 PEAK_HBM = 8000.0  # GB/s
 # HBM bytes per 4096-candidate render launch, measured with rocprofv3 --pmc:
-# FETCH_SIZE 137480 KiB (x 2, the factor a 1 GiB copy calibrates) in and
+# FETCH_SIZE 140300 KiB (x 2, the factor a 1 GiB copy calibrates) in and
 # WRITE_SIZE 1048576 KiB (x 1) out, i.e. the 4096 model images
-TRAFFIC_RENDER_4096 = (137480 * 2 + 1048576) * 1024
+TRAFFIC_RENDER_4096 = (140300 * 2 + 1048576) * 1024
 
 
 def make_workload(K):
