@@ -59,7 +59,7 @@ def build(verbose=True):
              "--offload-arch=" + ARCH, "-O3"] + NO_PACKED_FP32 + common)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    run([hipcc(), "-shared", "-o", tmp] + objs + ["-L/opt/rocm/lib", "-lhiprtc", "-lamdhip64", "-ldl",
+    run([hipcc(), "-shared", "-o", tmp] + objs + ["-L/opt/rocm/lib", "-lhiprtc", "-lamdhip64", "-ldl", "-lpthread",
                                                    "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB

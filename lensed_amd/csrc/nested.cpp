@@ -29,6 +29,7 @@
 #include <map>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/lensed.h"
@@ -541,16 +542,34 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
                 // proposal; either is a uniform draw over a superset of the
                 // constrained region, which is all the acceptance test needs.
                 const bool use_cube = bound.logvol_union >= 0;
-                double x[64];
-                for (int k = 0; k < b; ++k) {
-                    double* c = &cubes[(size_t)k * ndim];
-                    bool got = false;
-                    for (int tries = 0; !use_cube && !got && tries < 1000; ++tries) {
-                        bound.sample(rng, x);
-                        got = inside(x, ndim, o.wrap, c);
+                // kStreams independent streams seeded from the main one fill
+                // the batch (candidate k from stream k % kStreams), on threads
+                // for large batches; the draws do not depend on the thread count
+                constexpr int kStreams = 8;
+                unsigned long long seeds[kStreams];
+                for (auto& sd : seeds) sd = rng();
+                auto fill = [&](int st) {
+                    std::mt19937_64 g(seeds[st]);
+                    std::uniform_real_distribution<double> u01(0, 1);
+                    double x[64];
+                    for (int k = st; k < b; k += kStreams) {
+                        double* c = &cubes[(size_t)k * ndim];
+                        bool got = false;
+                        for (int tries = 0; !use_cube && !got && tries < 1000; ++tries) {
+                            bound.sample(g, x);
+                            got = inside(x, ndim, o.wrap, c);
+                        }
+                        if (!got)
+                            for (int i = 0; i < ndim; ++i) c[i] = u01(g);
                     }
-                    if (!got)
-                        for (int i = 0; i < ndim; ++i) c[i] = uni(rng);
+                };
+                if (b >= 512) {
+                    std::vector<std::thread> pool;
+                    for (int st = 1; st < kStreams; ++st) pool.emplace_back(fill, st);
+                    fill(0);
+                    for (auto& t : pool) t.join();
+                } else {
+                    for (int st = 0; st < kStreams; ++st) fill(st);
                 }
                 fn(cubes.data(), b, phys.data(), ll.data(), ctx);
                 r.nevals += b;
