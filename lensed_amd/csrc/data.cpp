@@ -114,16 +114,33 @@ bool parse_hdus(const std::vector<unsigned char>& f, std::vector<Hdu>& hdus, std
         }
         h.bitpix = (int)h.get_long("BITPIX", 0);
         long naxis = h.get_long("NAXIS", 0);
+        long pcount = h.get_long("PCOUNT", 0), gcount = h.get_long("GCOUNT", 1);
+        // sizes beyond what a file of this length can hold are malformed;
+        // checking each factor keeps the products below from overflowing
+        const size_t lim = f.size() + 1;
+        if (naxis < 0 || naxis > 999 || pcount < 0 || gcount < 0 || (size_t)pcount > lim || (size_t)gcount > lim ||
+            std::abs(h.bitpix) > 64) {
+            err = "malformed FITS header";
+            return false;
+        }
         size_t n = naxis > 0 ? 1 : 0;
         for (long i = 1; i <= naxis; ++i) {
             long v = h.get_long(("NAXIS" + std::to_string(i)).c_str(), 0);
+            if (v < 0 || (size_t)v > lim || (v > 0 && n > lim / (size_t)v)) {
+                err = "malformed FITS header";
+                return false;
+            }
             h.naxes.push_back(v);
-            n *= (size_t)std::max(0L, v);
+            n *= (size_t)v;
         }
         std::string xt = h.get_string("XTENSION");
         h.is_image = hdus.empty() || xt == "IMAGE";
-        long pcount = h.get_long("PCOUNT", 0), gcount = h.get_long("GCOUNT", 1);
-        size_t bytes = (size_t)std::abs(h.bitpix) / 8 * (size_t)gcount * ((size_t)pcount + n);
+        size_t units = (size_t)pcount + n;
+        if (gcount > 0 && units > lim / (size_t)gcount) {
+            err = "malformed FITS header";
+            return false;
+        }
+        size_t bytes = (size_t)std::abs(h.bitpix) / 8 * (size_t)gcount * units;
         h.data_offset = pos;
         h.data_bytes = n * (size_t)std::abs(h.bitpix) / 8;
         pos += (bytes + kRecord - 1) / kRecord * kRecord;

@@ -166,3 +166,29 @@ def test_data_from_the_configuration(tmp_path):
     assert obs.image.shape == (60, 80) and obs.pcs == (11.0, 21.0, 1.0, 1.0)
     assert (obs.image == full[20:80, 10:90]).all()
     assert abs(obs.weight - 4 / (obs.image + 0.5)).max() < 1e-6
+
+
+def test_random_input_is_an_error_not_a_crash(tmp_path):
+    from hypothesis import given, settings, strategies as st
+    alphabet = st.sampled_from(list("abcdefghijklmnopqrstuvwxyz.=:;#[] \t0123456789-_\n"))
+    p = tmp_path / "fuzz.ini"
+
+    @settings(max_examples=200, deadline=None, derandomize=True)
+    @given(st.lists(alphabet, max_size=300).map("".join), st.lists(st.text(max_size=30), max_size=3))
+    def run(text, args):
+        p.write_text("[objects]\nlens = sis\n" + text)
+        try:
+            cfg = C.Config(str(p))
+            for a in args:
+                try:
+                    cfg.arg("--" + a)
+                except C.ConfigError:
+                    pass
+            try:
+                cfg.check()
+            except C.ConfigError:
+                pass
+        except C.ConfigError:
+            pass
+
+    run()

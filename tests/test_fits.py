@@ -197,3 +197,45 @@ def test_observation_weights_mask_psf(tmp_path):
         D.Observation(t("img"), weight=t("psf"))  # wrong dimensions
     with pytest.raises(D.DataError):
         D.Observation(t("img"))  # neither weight nor gain
+
+
+def test_malformed_files_are_errors_not_crashes(tmp_path):
+    """Truncations, corrupted header values and random bytes: every outcome
+    is an image or a DataError (the reader checks sizes against the file)."""
+    from hypothesis import given, settings, strategies as st
+
+    a = np.arange(12 * 9, dtype=np.float32).reshape(9, 12)
+    good = str(tmp_path / "good.fits")
+    np_write_fits(good, [(None, 16, {}), (a, -32, {"EXTNAME": "IMG"})])
+    raw = open(good, "rb").read()
+    path = str(tmp_path / "bad.fits")
+
+    def attempt(data, spec=""):
+        with open(path, "wb") as f:
+            f.write(data)
+        try:
+            D.read_fits(path + spec)
+        except D.DataError:
+            pass
+
+    @settings(max_examples=150, deadline=None, derandomize=True)
+    @given(st.integers(0, len(raw)), st.lists(st.tuples(st.integers(0, len(raw) - 1), st.integers(0, 255)),
+                                               max_size=8))
+    def mutate(cut, flips):
+        b = bytearray(raw)
+        for i, v in flips:
+            b[i] = v
+        attempt(bytes(b[:cut]))
+        attempt(bytes(b), "[1]")
+
+    mutate()
+    for key, val in (("NAXIS1", "99999999999"), ("NAXIS2", "-5"), ("NAXIS", "1000"), ("BITPIX", "13"),
+                     ("GCOUNT", "4611686018427387904"), ("PCOUNT", "-1"), ("NAXIS1", "9223372036854775807")):
+        text = raw.decode("latin-1")
+        i = text.index(key.ljust(8) + "=", 2880)
+        card = (key.ljust(8) + "= " + val.rjust(20)).ljust(80)
+        bad = (text[:i] + card + text[i + 80:]).encode("latin-1")
+        with open(path, "wb") as f:
+            f.write(bad)
+        with pytest.raises(D.DataError):
+            D.read_fits(path + "[1]")
