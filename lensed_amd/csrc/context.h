@@ -112,6 +112,8 @@ struct lensed {
     std::map<std::vector<size_t>, lsd::Layout> layouts;
     std::string source;          // generated program (for dumps)
     std::string output;          // FITS file the dumper writes (empty: none)
+    void* d_npre = nullptr;      // per candidate and node: EPL lens-frame pair of the node offset
+    size_t cap_npre = 0;
     // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem
     hipEvent_t ev[9] = {};
     double prof_ms[8] = {};

@@ -36,6 +36,7 @@ struct lensed_desc {
     int npart, table_mode, item_e, pad4;  // chi^2 partials per candidate (8-row x 64-column blocks); EPL table
                                           // in use; first emulator node item (2 (NT + 1) nodes per candidate)
     long long psft_off;                  // floats into psf[]: the taps transposed, [psfw][psfh]
+    long long npre_off;                  // float4 into npre[]: the nodes' EPL lens-frame pairs ([ncand][nq])
 };
 
 // LDS row stride of the conv tile: 64 + PW - 1 columns plus room for the

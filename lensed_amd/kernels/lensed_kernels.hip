@@ -37,10 +37,25 @@ __device__ inline int lensed_find_sys(const lensed_desc* __restrict__ sys, int n
 
 // --------------------------------------------------------------- set_params
 // One thread per candidate: derived object constants from the physical
-// parameters, written as a data block of LENSED_DWORDS words.
+// parameters, written as a data block of LENSED_DWORDS words; with an EPL
+// table, also the lens-frame pair (F q_n, S q_n) of every quadrature node
+// offset, which the render kernel adds to the pixel's own part.
+template <int K>
+__device__ inline void lensed_node_pairs(const lensed_desc& d, u32* blk, const float4* __restrict__ quad,
+                                         float4* __restrict__ npre, int cand)
+{
+    const float4* q = quad + d.quad_off;
+    float4* out = npre + d.npre_off + (long long)cand * d.nq;
+    for (int n = 0; n < d.nq; ++n) {
+        lcl::float4 v = lcl::epl_pre_model<K>(blk, lcl::float2(q[n].x, q[n].y), false);
+        out[n] = make_float4(v.x, v.y, v.z, v.w);
+    }
+}
+
 extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
-    const float* __restrict__ params, u32* __restrict__ blocks)
+    const float* __restrict__ params, u32* __restrict__ blocks, const float4* __restrict__ quad,
+    float4* __restrict__ npre)
 {
     int item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= nitems) return;
@@ -52,7 +67,11 @@ extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
     for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = 0u;
     const float* p = params + d.param_off + (long long)cand * d.npar;
     switch (d.model) {
-#define LENSED_SET_CASE(K) case K: lcl::set_params_##K(blk, p); break;
+#define LENSED_SET_CASE(K)                                                \
+    case K:                                                               \
+        lcl::set_params_##K(blk, p);                                      \
+        if (d.table_mode) lensed_node_pairs<K>(d, blk, quad, npre, cand); \
+        break;
         LENSED_FOR_MODELS(LENSED_SET_CASE)
 #undef LENSED_SET_CASE
         default: break;
@@ -70,8 +89,8 @@ extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
 template <int K, bool ERR>
 __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int tile,
                                           const u32* __restrict__ blocks, const float4* __restrict__ quad,
-                                          const float4* __restrict__ tables, float4* tab_lds,
-                                          float* __restrict__ model, float* __restrict__ errp)
+                                          const float4* __restrict__ tables, const float4* __restrict__ npre,
+                                          float4* tab_lds, float* __restrict__ model, float* __restrict__ errp)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
@@ -79,18 +98,11 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
     for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = i < d.dwords ? __builtin_amdgcn_readfirstlane(gb[i]) : 0u;
     const float4* q = quad + d.quad_off;
     const lcl::float4* tab = nullptr;
-    const lcl::float4* npre = nullptr;
-    if (d.table_mode) {
-        // the candidate's EPL angular table, then the lens-frame offsets of
-        // the quadrature nodes (F q_n, S q_n), staged in LDS
+    if (d.table_mode) {  // the candidate's EPL angular table, staged in LDS
         const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
         for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
-        lcl::float4* pre_lds = (lcl::float4*)(tab_lds + LENSED_EPL_TABLE_F4);
-        for (int n = threadIdx.x; n < d.nq; n += LENSED_RENDER_BS)
-            pre_lds[n] = lcl::epl_pre_model<K>(blk, lcl::float2(q[n].x, q[n].y), false);
         __syncthreads();
         tab = (const lcl::float4*)tab_lds;
-        npre = pre_lds;
     }
     int k = tile * LENSED_RENDER_BS + (int)threadIdx.x;
     if (k < d.npix) {
@@ -102,11 +114,12 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
         float f = 0.0f, e = 0.0f;
         if (d.table_mode) {
             // F (p - c) + F q_n: the lens-frame pair of each ray from the
-            // pixel's and the node's parts
+            // pixel's part and the node's (from set_params, scalar loads)
             const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
+            const float4* np_c = npre + d.npre_off + (long long)cand * d.nq;
             for (int n = 0; n < d.nq; ++n) {
                 const float4 node = q[n];
-                const lcl::float4 np = npre[n];
+                const float4 np = np_c[n];
                 const lcl::float4 pre(base.x + np.x, base.y + np.y, base.z + np.z, base.w + np.w);
                 float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y), tab, pre);
                 f += node.z * v;
@@ -131,7 +144,7 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const u32* __restrict__ blocks, const float4* __restrict__ quad, const float4* __restrict__ tables,
-    float* __restrict__ model, float* __restrict__ errp)
+    const float4* __restrict__ npre, float* __restrict__ model, float* __restrict__ errp)
 {
     extern __shared__ float4 tab_lds[];
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
@@ -143,8 +156,8 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
         switch (d.model) {
 #define LENSED_RENDER_CASE(K)                                                                            \
     case K:                                                                                              \
-        if (errp) lensed_render_item<K, true>(d, cand, tile, blocks, quad, tables, tab_lds, model, errp); \
-        else lensed_render_item<K, false>(d, cand, tile, blocks, quad, tables, tab_lds, model, errp);     \
+        if (errp) lensed_render_item<K, true>(d, cand, tile, blocks, quad, tables, npre, tab_lds, model, errp); \
+        else lensed_render_item<K, false>(d, cand, tile, blocks, quad, tables, npre, tab_lds, model, errp);     \
         break;
             LENSED_FOR_MODELS(LENSED_RENDER_CASE)
 #undef LENSED_RENDER_CASE
@@ -479,14 +492,15 @@ __device__ inline void lensed_conv_chi2_body(const lensed_desc* __restrict__ sys
     float* tile = (float*)tile4;
     __shared__ double red[LENSED_CONV_BS / 64];
     __shared__ float rows8[8 * LENSED_CONV_TX];
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    auto run = [&](int item) {
         int s = lensed_find_sys<1>(sys, nsys, item);
         const lensed_desc& d = sys[s];
         int local = item - d.item_c;
         int cand = local / d.ntile_c;
         int t = local - cand * d.ntile_c;
         lensed_conv_item<RY>(d, cand, t, model, image, weight, psf, partials, conv_out, chi2_out, tile, red, rows8);
-    }
+    };
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) run(item);
 }
 
 extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) void lensed_conv_chi2(
