@@ -259,7 +259,8 @@ typedef struct {
     int maxiter;     /* iteration limit (0 = none) */
     int resume;      /* continue from <root>resume.dat */
     int outfile;     /* write <root>.txt */
-    const int* wrap; /* npar periodic flags (sampler order), or NULL */
+    const int* wrap; /* npar periodic flags (sampler order), or NULL: none, or in
+                        lensed_nested_run() the parameters' own `wrap` flags */
     const char* root;/* output prefix, or NULL */
     int feedback;    /* progress lines on stderr every updint (or 1000) iterations */
 } lensed_ns_options;

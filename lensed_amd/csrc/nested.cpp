@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "../../include/lensed.h"
+#include "context.h"
 
 namespace lsd {
 void set_error(const std::string& msg);
@@ -465,6 +466,50 @@ void call_dumper(DumperFn dumper, void* ctx, Result& r, const std::vector<Point>
     dumper(&ns, &nl, &np, &pl, &po, &co, &maxl, &logz, &ins, &err, ctx);
 }
 
+// Constant efficiency mode (MultiNest's ceff): the enlargement is steered
+// once per bound, from that bound's acceptance, towards acceptance efr, by
+// at most a factor 2 a step; as in MultiNest the bound may then shrink below
+// the one that encloses the live points, which trades evidence accuracy for
+// speed.  Shrinking raises the acceptance where the constrained region is
+// compact (and its outermost live points stick out into thin tails), but
+// lowers it where the region is a thin or curved sheet that falls out of a
+// shrunken ellipsoid.  So when a step meant to raise the acceptance more
+// than halved it instead, or a limit stops the steps, the direction turns
+// round; the enlargement then stays near the one that accepts best.
+struct Steer {
+    double efr;
+    double acc_prev = -1;
+    bool raise_prev = false;  // the last step was meant to raise the acceptance
+    int dir = -1;             // the step that should raise it: shrink (-1) or grow (+1)
+
+    double operator()(double volfac, long acc_n, long prop_n, bool prior_draws) {
+        const double acc = (double)acc_n / prop_n;
+        const double expect = acc_prev * prop_n;
+        if (prior_draws)
+            dir = -1;  // the bound held the whole cube: only shrinking changes the draws
+        else if (raise_prev && acc_n + 2 * std::sqrt(expect) < 0.5 * expect)
+            dir = -dir;
+        double f;
+        bool raise = acc < efr;
+        if (!raise) {
+            // (no growing past a bound that already holds the whole cube)
+            f = prior_draws ? 1.0 : std::min(acc / efr, 2.0);
+            dir = -1;
+        } else {
+            const double g = std::max(acc / efr, 0.5);
+            f = dir < 0 ? g : 1 / g;
+        }
+        const double next = std::min(1e3, std::max(1e-2, volfac * f));
+        if (raise && next == volfac) {
+            dir = -dir;
+            raise = false;
+        }
+        acc_prev = acc;
+        raise_prev = raise;
+        return next;
+    }
+};
+
 int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn dumper, void* dctx, Result& r) {
     if (ndim <= 0 || ndim > 64 || o.nlive <= ndim) {
         lsd::set_error("nested sampling needs 0 < ndim < nlive and ndim <= 64");
@@ -493,16 +538,19 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
     double logx = shrink * r.niter;
     // enlargement: without ceff, efr is MultiNest's shrinking factor and the
     // bounding volume grows by 1/efr (by at least half); with ceff, efr is
-    // the target acceptance and the enlargement adapts to it, never below
-    // the tight bound of the live points
+    // the target acceptance and Steer adapts the enlargement to it
     double volfac = o.ceff ? 1.5 : std::max(1.5, 1.0 / o.efr);
     Bound bound;
     if (o.ceff) bound.boot = 0;
+    Steer steer{o.efr};
+    const double refit_prop = std::max(100.0, 4.0 * std::max(1, n / 10) / o.efr);
     long fb_acc = 0, fb_prop = 0;
     std::vector<double> cubes, phys, ll;
     size_t next = 0, have = 0;
     int since_fit = n;
-    long accepted = 0, proposed = 0;
+    long fit_acc = 0, fit_prop = 0;  // proposals drawn from the current bound
+    long b_acc = 0, b_prop = 0;      // from the current batch
+    double acc_last = 1.0;
     while (true) {
         int worst = 0;
         for (int i = 1; i < n; ++i)
@@ -528,11 +576,32 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
         bool done = false;
         while (!done) {
             if (next >= have) {
-                if (since_fit >= std::max(1, n / 10) || bound.ells.empty()) {
+                // The bound is refitted after n/10 replacements; in constant
+                // efficiency mode also after 4 x the proposals those should take
+                // at acceptance efr, so that a bound far too large or small is
+                // not kept while its few replacements trickle in.
+                if (bound.ells.empty() || since_fit >= std::max(1, n / 10) || (o.ceff && fit_prop >= refit_prop)) {
+                    if (o.ceff && fit_prop > 0) volfac = steer(volfac, fit_acc, fit_prop, bound.logvol_union >= 0);
                     bound.fit(live, volfac, logx, o.wrap, true, rng);
                     since_fit = 0;
+                    fit_acc = fit_prop = 0;
                 }
+                // A batch is drawn from one bound, which grows stale as the
+                // live points contract while the batch is used up (its
+                // acceptance falls, and in constant efficiency mode the
+                // enlargement would shrink to make up for it).  So a large
+                // batch is sized to what the next refit needs, 1.5 x n/10
+                // replacements at the last batch's acceptance, in powers of
+                // two from 256 (a few launch shapes) up to o.batch.
+                if (b_prop > 0) acc_last = (double)b_acc / b_prop;
+                b_acc = b_prop = 0;
                 int b = std::max(1, o.batch);
+                if (b > 256) {
+                    const double want = 1.5 * std::max(1, n / 10) / std::max(acc_last, 1e-4);
+                    int c = 256;
+                    while (c < b && c < want) c *= 2;
+                    b = std::min(b, c);
+                }
                 cubes.resize((size_t)b * ndim);
                 phys.resize((size_t)b * npar);
                 ll.resize(b);
@@ -577,36 +646,28 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
                 have = b;
             }
             size_t k = next++;
-            proposed += 1;
+            fit_prop += 1;
+            fb_prop += 1;
+            b_prop += 1;
             if (ll[k] > lmin) {
                 live[worst].u.assign(cubes.begin() + k * ndim, cubes.begin() + (k + 1) * ndim);
                 live[worst].phys.assign(phys.begin() + k * npar, phys.begin() + (k + 1) * npar);
                 live[worst].logl = ll[k];
-                accepted += 1;
+                fit_acc += 1;
+                fb_acc += 1;
+                b_acc += 1;
                 since_fit += 1;
                 done = true;
             }
-        }
-        // constant efficiency mode: steer the enlargement towards acceptance efr
-        // (as in MultiNest, the bound may then shrink below the one that
-        // encloses the live points, which trades evidence accuracy for speed)
-        if (proposed >= 100) {
-            if (o.ceff) {
-                double acc = (double)accepted / proposed;
-                volfac *= acc < o.efr ? 0.9 : 1.1;
-                volfac = std::min(1e3, std::max(1e-2, volfac));
-            }
-            fb_acc += accepted;
-            fb_prop += proposed;
-            accepted = proposed = 0;
         }
         if (o.feedback && r.niter % (o.updint > 0 ? o.updint : 1000) == 0)
         {
             fprintf(stderr,
                     "nested: iteration %ld  ln Z %.4f  ln X %.2f  evaluations %ld  ellipsoids %zu  ln V %.2f  "
-                    "acceptance %.2e%s\n",
+                    "acceptance %.2e  enlargement %.3g%s\n",
                     r.niter, r.logz, logx, r.nevals, bound.ells.size(), bound.logvol_union,
-                    fb_prop ? (double)fb_acc / fb_prop : 0.0, bound.logvol_union >= 0 ? "  (prior draws)" : "");
+                    fb_prop ? (double)fb_acc / fb_prop : 0.0, volfac,
+                    bound.logvol_union >= 0 ? "  (prior draws)" : "");
             fb_acc = fb_prop = 0;
         }
         if (o.updint > 0 && r.niter % o.updint == 0) {
@@ -724,6 +785,11 @@ int lensed_nested_run(lensed* l, size_t s, const lensed_ns_options* opts, double
     int ndim = (int)lensed_ndims(l, s), npar = (int)lensed_npars(l, s);
     BatchCtx b{l, s, ndim};
     Options o = to_options(opts, npar);
+    if (!opts || !opts->wrap) {  // the parameters' own periodic flags (sampler order), as Lensed passes them
+        const auto& sys = l->sys[s];
+        o.wrap.resize(npar);
+        for (int i = 0; i < npar; ++i) o.wrap[i] = sys.pars[sys.pmap[i]].wrap;
+    }
     Result r;
     if (sample(ndim, npar, lensed_batch, &b, o, s == 0 ? dumper : nullptr, l, r)) return -1;
     fill_result(r, result, mean, sigma, ml, npar);

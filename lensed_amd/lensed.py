@@ -200,6 +200,12 @@ class Context:
         L.load().lensed_pmap(self.ptr, s, out)
         return list(out)
 
+    def wrap(self, s=0):
+        """Periodic flags of the parameters in sampler order (what Lensed
+        passes to MultiNest's run(), src/lensed.c:1254-1259)."""
+        flags = [o.obj.pars[j].wrap for o in self.systems[s].objects for j in range(o.obj.npars)]
+        return [int(flags[k]) for k in self.pmap(s)]
+
     def loglike(self, cube):
         """One candidate through the MultiNest callback; returns (logL, physical)."""
         npar = self.npars(0)

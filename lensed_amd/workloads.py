@@ -77,6 +77,12 @@ class Workload:
             pri[k] = repr(float(self.truth[k]))
         return SystemSpec(self.objects, pri, self.image, self.weight, psf=self.psf)
 
+    def prior_range(self, key):
+        """(lower, upper) of a `[wrap] [image] unif lower upper` prior."""
+        words = [w for w in self.priors[key].split() if w not in ("wrap", "image")]
+        assert words[0] == "unif", self.priors[key]
+        return float(words[1]), float(words[2])
+
     def candidates(self, ndim, n=None):
         """Seeded unit-cube draws (MultiNest's input)."""
         rng = np.random.default_rng(self.cand_seed)
@@ -123,10 +129,10 @@ def c3():
              "source.x": 130.0, "source.y": 127.5, "source.sigma": 0.1 * ARCSEC, "source.mag": 0.0,
              "source.q": 0.8, "source.pa": 20.0}
     priors = {"lens.x": "unif 126 131", "lens.y": "unif 126 131", "lens.r": "unif 15 25",
-              "lens.t": "unif 0.5 1.5", "lens.q": "unif 0.3 1", "lens.pa": "unif 0 180",
+              "lens.t": "unif 0.5 1.5", "lens.q": "unif 0.3 1", "lens.pa": "wrap unif 0 180",
               "lens.g1": "unif -0.1 0.1", "lens.g2": "unif -0.1 0.1",
               "source.x": "unif 125 135", "source.y": "unif 122 132", "source.sigma": "unif 1.5 2.5",
-              "source.mag": "unif -5 5", "source.q": "unif 0.4 1", "source.pa": "unif 0 180"}
+              "source.mag": "unif -5 5", "source.q": "unif 0.4 1", "source.pa": "wrap unif 0 180"}
     return Workload("C3", objects, truth, priors, (256, 256), 0.05, psf=gaussian_psf(21, 3.0), seed=3,
                     cand_seed=103, ncand=4096)
 

@@ -111,10 +111,12 @@ def test_c3_emulated_batched_posterior_matches_plain_run():
     lib = L.load()
     with Context(wl.spec(), max_batch=1) as ctx:
         npar, ndim = ctx.npars(), ctx.ndims()
+        # position angles are periodic, as Lensed's examples declare them
+        assert [pids[k] for k, w in zip(ctx.pmap(), ctx.wrap()) if w] == ["lens.pa", "source.pa"]
         t0 = time.perf_counter()
         nested.run_multinest_abi(ctypes.cast(lib.loglike, L.LOGLIKE_FN), ndim, npar,
-                                 ctypes.cast(lib.dumper, L.DUMPER_FN), updint=100000, seed=31, context=ctx.ptr,
-                                 **kw)
+                                 ctypes.cast(lib.dumper, L.DUMPER_FN), updint=100000, seed=31, wrap=ctx.wrap(),
+                                 context=ctx.ptr, **kw)
         t_plain = time.perf_counter() - t0
         mean_p, sig_p, logz_p, logz_err_p, maxl_p = results(ctx)
     print(f"C3 plain run(): {t_plain:.1f} s, logZ {logz_p:.2f}, max logL {maxl_p:.2f}", flush=True)

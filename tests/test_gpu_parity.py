@@ -110,8 +110,7 @@ def test_c1_loglike_plain_batch_oracle(c1):
     with Context(wl.spec(), max_batch=64) as ctx:
         assert ctx.npars() == 12 and ctx.ndims() == 12
         cubes = wl.candidates(ctx.ndims(), 64)
-        cubes[0] = [(wl.truth[k] - lo) / (hi - lo) for k, (lo, hi) in
-                    ((k, map(float, wl.priors[k].split()[1:])) for k in pids)]
+        cubes[0] = [(wl.truth[k] - lo) / (hi - lo) for k, (lo, hi) in ((k, wl.prior_range(k)) for k in pids)]
         lb, pb = ctx.loglike_batch(cubes)
         for i in range(8):
             lp, pp = ctx.loglike(cubes[i])

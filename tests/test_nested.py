@@ -103,6 +103,47 @@ def test_multimodal_and_curved_evidence(logl):
     assert r.nevals / r.niter < 20
 
 
+def correlated_gaussian(d=8):
+    rng = np.random.default_rng(7)
+    c = rng.uniform(0.35, 0.65, d)
+    a = rng.normal(size=(d, d))
+    cov = a @ a.T + d * np.eye(d)
+    sd = rng.uniform(0.004, 0.02, d)
+    cov = cov / np.sqrt(np.outer(np.diag(cov), np.diag(cov))) * np.outer(sd, sd)
+    logz = 0.5 * np.linalg.slogdet(2 * np.pi * cov)[1]
+    return c, sd, np.linalg.inv(cov), logz
+
+
+@pytest.mark.parametrize("batch", [1, 1024])
+def test_constant_efficiency_mode(batch):
+    """Lensed's default sampler settings (ceff on, target acceptance 0.05,
+    src/input/options.c:150-227): the acceptance settles at the target while
+    the moments and, for this compact region, the evidence stay right."""
+    c, sd, prec, logz = correlated_gaussian()
+    r = nested.sample(lambda u: -0.5 * np.einsum("ij,jk,ik->i", u - c, prec, u - c), len(c), nlive=300, tol=0.1,
+                      efr=0.05, ceff=True, batch=batch, seed=2)
+    assert 0.04 < r.niter / r.nevals < 0.07, r.niter / r.nevals
+    assert np.all(np.abs(r.mean - c) < 0.15 * sd), (r.mean - c) / sd
+    assert np.all(np.abs(r.sigma / sd - 1) < 0.1), r.sigma / sd
+    assert abs(r.logz - logz) < 4 * r.logz_err, (r.logz, logz, r.logz_err)
+
+
+def test_constant_efficiency_thin_shell():
+    """A thin spherical shell in 5-D: shrinking the bound towards its centre
+    leaves the region, so steering the enlargement only ever downwards for
+    more acceptance ends at almost none (such a run stalled at millions of
+    evaluations per iteration).  Evidence: the shell's area x sqrt(2 pi) w."""
+    d, radius, w = 5, 0.3, 0.002
+    logz = math.log(2 * math.pi ** (d / 2) / math.gamma(d / 2) * radius ** (d - 1) * math.sqrt(2 * math.pi) * w)
+
+    def shell(u):
+        return -0.5 * ((np.sqrt(((u - 0.5) ** 2).sum(1)) - radius) / w) ** 2
+    r = nested.sample(shell, d, nlive=400, tol=0.1, efr=0.05, ceff=True, batch=1024, seed=1)
+    assert r.nevals / r.niter < 200, r.nevals / r.niter
+    assert np.all(np.abs(r.mean - 0.5) < 0.02)
+    assert abs(r.logz - logz) < 4 * r.logz_err, (r.logz, logz, r.logz_err)
+
+
 def test_multinest_abi_run_and_dumper():
     """run() with MultiNest's callbacks: cube is unit in, physical out."""
     ndim, npar = 2, 3
