@@ -233,7 +233,8 @@ static std::string data_ptr(const std::string& m, size_t off, const char* base) 
 std::string compute_function(const std::vector<ModelObject>& objs, int k) {
     std::ostringstream o;
     // a tabulated EPL lens (at most one per model) starts from the pair
-    // (F (x - c), S (x - c)) that the render kernel forms incrementally
+    // (F (x - c), x - S (x - c)) that the render kernel forms incrementally
+    // (pixel part at d = p, node part at d = q_n)
     o << "static inline float4 epl_pre_" << k << "(uint* data, float2 d, bool pixel)\n{\n"
       << "    (void)data; (void)d; (void)pixel;\n";
     {
@@ -244,7 +245,7 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
                 std::string m = mangle(ob.name);
                 o << "    const struct data_" << m << "* s = " << data_ptr(m, dd, "data") << ";\n"
                   << "    return lensed_epl_pre" << (ob.name == "epl_plus_shear" ? "_shear" : "")
-                  << "(s, pixel ? d - s->c : d);\n";
+                  << "(s, pixel ? d - s->c : d, d);\n";
                 any = true;
             }
             dd += ob.size;
@@ -257,11 +258,17 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
       << "    (void)tab; (void)pre;\n"
       << "    float2 y = x;\n"
       << "    float f = 0;\n";
-    bool plane_open = false;
+    bool plane_open = false, lin = false;
     int trigger = 0;
     size_t d = 0;
+    // with a tabulated EPL lens, pre.zw is the ray position with the lens's
+    // linear (shear) deflection already taken off, x - S (x - c); the rest
+    // of the deflection is subtracted from it.  The guard then sees the
+    // deflection without the shear, which is finite wherever the shear is
+    // (finite parameters), so the rays it rejects are the same.
     auto close_plane = [&]() {
-        o << "        y -= lensed_deflection_ok(a) ? a : float2(1E10f, 1E10f);\n    }\n";
+        o << "        y = " << (lin ? "float2(pre.z, pre.w)" : "y")
+          << " - (lensed_deflection_ok(a) ? a : float2(1E10f, 1E10f));\n    }\n";
         plane_open = false;
     };
     for (const auto& ob : objs) {
@@ -275,10 +282,10 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
                 o << "    {\n        float2 a = 0;\n";
                 plane_open = true;
             }
-            if (ob.table)
-                o << "        a += lensed_epl_table_deflection" << (ob.name == "epl_plus_shear" ? "_shear" : "")
-                  << "_pre(" << data_ptr(m, d, "data") << ", pre, tab);\n";
-            else
+            if (ob.table) {
+                o << "        a += lensed_epl_table_deflection_pre(" << data_ptr(m, d, "data") << ", pre, tab);\n";
+                lin = true;
+            } else
                 o << "        a += deflection_" << m << "(" << data_ptr(m, d, "data") << ", y);\n";
         } else if (ob.type == OBJ_SOURCE) {
             o << "    f += brightness_" << m << "(" << data_ptr(m, d, "data") << ", y);\n";

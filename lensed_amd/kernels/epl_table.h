@@ -16,11 +16,14 @@
 // The table of a candidate is built from the series (or by the emulator)
 // and staged in LDS by the render kernel.
 //
-// The lens-frame position u = F (x - c) and the shear term S (x - c) are
-// linear in the ray position x = p + q_n (pixel centre plus quadrature
-// node), so the render kernel forms them as F (p - c) + F q_n with F q_n
-// computed once per node and candidate (lensed_epl_pre*); the deflection
-// then starts from the pair (u, S (x - c)).
+// The lens-frame position u = F (x - c) and the ray position with the
+// lens's linear (external shear) deflection already taken off, x - S (x - c),
+// are linear in the ray position x = p + q_n (pixel centre plus quadrature
+// node), so the render kernel forms both as a pixel part plus a node part,
+// the node parts computed once per node and candidate (lensed_epl_pre*).
+// The generated ray code then subtracts only the tabulated EPL deflection
+// (and any other lens's) from the second pair: the shear costs nothing per
+// ray.
 #pragma once
 
 // LENSED_EPL_NT (intervals per octant), LENSED_EPL_ROWS and
@@ -54,44 +57,30 @@ inline float2 lensed_epl_omega_tab(float2 u, const float4* tab) {
     return float2(copysign(ox, u.x), copysign(oy, u.y));
 }
 
-// (F d, 0, 0) and (F d, S d): the lens-frame position and the shear term of
-// an offset d; D is the object's data struct (fields c, fwd, bwd, [shear],
-// hexp; bwd carries the normalisation)
+// the pair of the offset r = x - c at position x: (F r, x) without shear,
+// (F r, x - S r) with it; both are linear in (r, x), so the pixel part
+// (r = p - c, x = p) and the node part (r = x = q_n) add up to the ray's.
+// D is the object's data struct (fields c, fwd, bwd, [shear], hexp; bwd
+// carries the normalisation).
 template <class D>
-inline float4 lensed_epl_pre(const D* self, float2 d) {
-    float2 u = mv22(self->fwd, d);
-    return float4(u.x, u.y, 0.0f, 0.0f);
+inline float4 lensed_epl_pre(const D* self, float2 r, float2 x) {
+    float2 u = mv22(self->fwd, r);
+    return float4(u.x, u.y, x.x, x.y);
 }
 
 template <class D>
-inline float4 lensed_epl_pre_shear(const D* self, float2 d) {
-    float2 u = mv22(self->fwd, d), g = mv22(self->shear, d);
+inline float4 lensed_epl_pre_shear(const D* self, float2 r, float2 x) {
+    float2 u = mv22(self->fwd, r), g = x - mv22(self->shear, r);
     return float4(u.x, u.y, g.x, g.y);
 }
 
-// the deflection from the pair; pre = (u, S (x - c))
+// the EPL deflection (without the shear) from the pair
 template <class D>
 inline float2 lensed_epl_table_deflection_pre(const D* self, float4 pre, const float4* tab) {
     float2 u = float2(pre.x, pre.y);
     float uu = dot(u, u);
     float2 om = lensed_epl_omega_tab(u, tab);
     return mv22(self->bwd, powr(uu, self->hexp) * om);
-}
-
-template <class D>
-inline float2 lensed_epl_table_deflection_shear_pre(const D* self, float4 pre, const float4* tab) {
-    return lensed_epl_table_deflection_pre(self, pre, tab) + float2(pre.z, pre.w);
-}
-
-// the same from a position
-template <class D>
-inline float2 lensed_epl_table_deflection(const D* self, float2 x, const float4* tab) {
-    return lensed_epl_table_deflection_pre(self, lensed_epl_pre(self, x - self->c), tab);
-}
-
-template <class D>
-inline float2 lensed_epl_table_deflection_shear(const D* self, float2 x, const float4* tab) {
-    return lensed_epl_table_deflection_shear_pre(self, lensed_epl_pre_shear(self, x - self->c), tab);
 }
 
 }  // namespace lcl

@@ -126,8 +126,9 @@ extern "C" void host_omega(double q, double t, const float* u, int n, float* out
         out[2 * k] = o.x; out[2 * k + 1] = o.y;
     }
 }
-// deflection of object 0 (epl_plus_shear) at pixel p plus node q, from the split pair
-extern "C" void host_deflect(const float* params, const float* p, const float* qn, int n, float* out) {
+// source-plane position of the ray at pixel p plus node q behind object 0
+// (epl_plus_shear), from the split pair as the generated ray code forms it
+extern "C" void host_trace(const float* params, const float* p, const float* qn, int n, float* out) {
     static lcl::uint data[4096];
     lcl::set_params_0(data, params);
     const auto* s = (const lcl::data_epl_plus_shear*)data;
@@ -135,10 +136,10 @@ extern "C" void host_deflect(const float* params, const float* p, const float* q
     build_table(params[4], params[3], tab);
     for (int k = 0; k < n; ++k) {
         lcl::float2 px(p[2 * k], p[2 * k + 1]), qq(qn[2 * k], qn[2 * k + 1]);
-        lcl::float4 b = lcl::lensed_epl_pre_shear(s, px - s->c), d = lcl::lensed_epl_pre_shear(s, qq);
+        lcl::float4 b = lcl::lensed_epl_pre_shear(s, px - s->c, px), d = lcl::lensed_epl_pre_shear(s, qq, qq);
         lcl::float4 pre(b.x + d.x, b.y + d.y, b.z + d.z, b.w + d.w);
-        lcl::float2 a = lcl::lensed_epl_table_deflection_shear_pre(s, pre, tab);
-        out[2 * k] = a.x; out[2 * k + 1] = a.y;
+        lcl::float2 y = lcl::float2(pre.z, pre.w) - lcl::lensed_epl_table_deflection_pre(s, pre, tab);
+        out[2 * k] = y.x; out[2 * k + 1] = y.y;
     }
 }
 '''
@@ -159,7 +160,7 @@ def _host_lib():
     h = ctypes.CDLL(so)
     fp = POINTER(c_float)
     h.host_omega.argtypes = [c_double, c_double, fp, c_int, fp]
-    h.host_deflect.argtypes = [fp, fp, fp, c_int, fp]
+    h.host_trace.argtypes = [fp, fp, fp, c_int, fp]
     return h
 
 
@@ -185,7 +186,9 @@ def test_product_table_lookup_on_the_host():
     assert np.all(np.isfinite(z))
 
 
-def test_split_pair_deflection_matches_oracle():
+def test_split_pair_ray_tracing_matches_oracle():
+    """The source-plane position x - S (x - c) - a_EPL(x) of a ray formed from
+    pixel and node parts, against x - alpha(x) of the FP64 oracle."""
     h = _host_lib()
     rng = np.random.default_rng(5)
     fp = POINTER(c_float)
@@ -194,8 +197,9 @@ def test_split_pair_deflection_matches_oracle():
     p = rng.uniform(30, 70, (2000, 2)).astype(np.float32)
     qn = rng.uniform(-0.5, 0.5, (2000, 2)).astype(np.float32)
     out = np.zeros_like(p)
-    h.host_deflect(params.astype(np.float32).ctypes.data_as(fp), p.ctypes.data_as(fp), qn.ctypes.data_as(fp),
-                   len(p), out.ctypes.data_as(fp))
-    exact = O.deflect(objs, params, 0, p.astype(np.float64) + qn.astype(np.float64))
-    scale = np.max(np.abs(exact))
-    assert np.max(np.abs(out - exact)) < 2e-5 * scale, np.max(np.abs(out - exact)) / scale
+    h.host_trace(params.astype(np.float32).ctypes.data_as(fp), p.ctypes.data_as(fp), qn.ctypes.data_as(fp),
+                 len(p), out.ctypes.data_as(fp))
+    x = p.astype(np.float64) + qn.astype(np.float64)
+    alpha = O.deflect(objs, params, 0, x)
+    scale = np.max(np.abs(alpha))
+    assert np.max(np.abs(out - (x - alpha))) < 2e-5 * scale, np.max(np.abs(out - (x - alpha))) / scale
