@@ -82,6 +82,10 @@ char* kernel_name(const char* prefix, const char* name);
  * plugin translation and the generated ray-shoot on the CPU.  malloc'd,
  * NULL on failure. */
 char* lensed_model_source(size_t nobjs, const object* objs);
+/* The same with the first EPL lens tabulated, as the table and emulator
+ * modes generate it (epl_pre_0() and the table form of compute_0()).  No
+ * counterpart in the reference; for the same CPU checks. */
+char* lensed_model_source_tabulated(size_t nobjs, const object* objs);
 
 /* Build check with no counterpart in the reference (it compiles in
  * clBuildProgram, src/lensed.c:755): compile a program from main_program()

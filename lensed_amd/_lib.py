@@ -75,6 +75,7 @@ SIGNATURES = {
     "kernel_name": (c_void_p, [c_char_p, c_char_p]),
     "lensed_program_compile": (c_long, [c_char_p, c_char_p, POINTER(c_void_p)]),
     "lensed_model_source": (c_void_p, [c_size_t, POINTER(Object)]),
+    "lensed_model_source_tabulated": (c_void_p, [c_size_t, POINTER(Object)]),
     "prior_read": (c_void_p, [c_char_p]),
     "prior_default": (c_void_p, [c_double]),
     "prior_free": (None, [c_void_p]),

@@ -157,6 +157,22 @@ char* lensed_model_source(size_t nobjs, const object* objs) {
     return dup(src);
 }
 
+char* lensed_model_source_tabulated(size_t nobjs, const object* objs) {
+    std::vector<ModelObject> model = model_of(nobjs, objs);
+    for (auto& mo : model)
+        if (mo.name == "epl" || mo.name == "epl_plus_shear") {
+            mo.table = true;
+            break;
+        }
+    std::string err;
+    std::string src = model_source({model}, &err);
+    if (src.empty()) {
+        set_error(err);
+        return nullptr;
+    }
+    return dup(src);
+}
+
 const char* kernel_options(size_t width, size_t height, int psf, size_t psfw, size_t psfh, size_t nq,
                            const char* flags[]) {
     char buf[256];

@@ -17,6 +17,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <map>
+#include <mutex>
+#include <regex>
 #include <sstream>
 
 namespace lsd {
@@ -121,12 +124,20 @@ std::string translate_object(const std::string& src, const std::string& mangled)
     return out;
 }
 
-static const char* const kObjectMacros[] = {"type", "params", "data", "deflection", "brightness", "foreground", "set"};
+static const char* const kObjectMacros[] = {"type",       "params",       "data",         "deflection",   "brightness",
+                                             "foreground", "set",          "frame_centre", "frame_map",    "frame_profile"};
 
 static std::string undef_macros() {
     std::string s;
     for (const char* m : kObjectMacros) s += std::string("#undef ") + m + "\n";
     return s;
+}
+
+// a source may describe itself in its own frame u = frame_map(x - frame_centre)
+// with frame_map linear: brightness(x) = frame_profile(u)
+static std::string frame_macros(const std::string& m) {
+    return "#define frame_centre frame_centre_" + m + "\n#define frame_map frame_map_" + m +
+           "\n#define frame_profile frame_profile_" + m + "\n";
 }
 
 std::string wrap_object(const std::string& translated, const std::string& m, const std::string& name) {
@@ -139,6 +150,7 @@ std::string wrap_object(const std::string& translated, const std::string& m, con
       << "#define brightness brightness_" << m << "\n"
       << "#define foreground foreground_" << m << "\n"
       << "#define set set_" << m << "\n"
+      << frame_macros(m)
       << "#line 1 \"objects/" << name << ".cl\"\n"
       << translated << "\n"
       << undef_macros();
@@ -154,6 +166,7 @@ std::string wrap_object_meta(const std::string& translated, const std::string& m
       << "#define brightness brightness_" << m << "\n"
       << "#define foreground foreground_" << m << "\n"
       << "#define set set_" << m << "\n"
+      << frame_macros(m)
       << "#line 1 \"objects/" << name << ".cl\"\n"
       << translated << "\n"
       << undef_macros()
@@ -230,27 +243,73 @@ static std::string data_ptr(const std::string& m, size_t off, const char* base) 
     return o.str();
 }
 
+static bool source_has_frame(const std::string& name) {
+    static std::mutex mu;
+    static std::map<std::string, bool> known;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = known.find(name);
+    if (it != known.end()) return it->second;
+    std::string src;
+    bool ok = load_object_source(name, &src, nullptr);
+    for (const char* re : {R"(static\s+float2\s+frame_centre\s*\()", R"(static\s+float2\s+frame_map\s*\()",
+                           R"(static\s+float\s+frame_profile\s*\()"})
+        ok = ok && std::regex_search(src, std::regex(re));
+    return known[name] = ok;
+}
+
+// The lens-frame form of a tabulated EPL lens applies when that lens is the
+// only one and a single source behind it has a frame (source_has_frame):
+// its index, or -1.
+static int frame_source(const std::vector<ModelObject>& objs) {
+    int nlens = 0, table = -1, nbehind = 0, src = -1;
+    for (size_t i = 0; i < objs.size(); ++i) {
+        if (objs[i].type == OBJ_LENS) {
+            ++nlens;
+            if (objs[i].table) table = (int)i;
+        } else if (objs[i].type == OBJ_SOURCE && nlens > 0) {
+            ++nbehind;
+            src = (int)i;
+        }
+    }
+    if (nlens != 1 || table < 0 || nbehind != 1 || !source_has_frame(objs[src].name)) return -1;
+    return src;
+}
+
 std::string compute_function(const std::vector<ModelObject>& objs, int k) {
     std::ostringstream o;
-    // a tabulated EPL lens (at most one per model) starts from the pair
-    // (F (x - c), x - S (x - c)) that the render kernel forms incrementally
-    // (pixel part at d = p, node part at d = q_n)
-    o << "static inline float4 epl_pre_" << k << "(uint* data, float2 d, bool pixel)\n{\n"
-      << "    (void)data; (void)d; (void)pixel;\n";
+    const int fsrc = frame_source(objs);
+    size_t lens_off = 0, src_off = 0;
+    std::string lm, sm, pre_fn = "lensed_epl_pre";
     {
         size_t dd = 0;
-        bool any = false;
-        for (const auto& ob : objs) {
-            if (ob.table && !any) {
-                std::string m = mangle(ob.name);
-                o << "    const struct data_" << m << "* s = " << data_ptr(m, dd, "data") << ";\n"
-                  << "    return lensed_epl_pre" << (ob.name == "epl_plus_shear" ? "_shear" : "")
-                  << "(s, pixel ? d - s->c : d, d);\n";
-                any = true;
+        for (size_t i = 0; i < objs.size(); ++i) {
+            if (objs[i].table && lm.empty()) {
+                lens_off = dd;
+                lm = mangle(objs[i].name);
+                if (objs[i].name == "epl_plus_shear") pre_fn += "_shear";
             }
-            dd += ob.size;
+            if ((int)i == fsrc) src_off = dd, sm = mangle(objs[i].name);
+            dd += objs[i].size;
         }
-        if (!any) o << "    return float4(0.0f, 0.0f, 0.0f, 0.0f);\n";
+    }
+    // a tabulated EPL lens (at most one per model) starts from the pair
+    // (F (x - c), x - S (x - c)) that the render kernel forms incrementally
+    // (pixel part at d = p, node part at d = q_n); in the lens-frame form
+    // the second half is taken on into the source's frame, T (x - S (x - c) - c_s)
+    o << "static inline float4 epl_pre_" << k << "(uint* data, float2 d, bool pixel)\n{\n"
+      << "    (void)data; (void)d; (void)pixel;\n";
+    if (!lm.empty()) {
+        o << "    const struct data_" << lm << "* s = " << data_ptr(lm, lens_off, "data") << ";\n"
+          << "    const float4 e = " << pre_fn << "(s, pixel ? d - s->c : d, d);\n";
+        if (fsrc >= 0)
+            o << "    struct data_" << sm << "* g = " << data_ptr(sm, src_off, "data") << ";\n"
+              << "    const float2 X = float2(e.z, e.w);\n"
+              << "    const float2 P = frame_map_" << sm << "(g, pixel ? X - frame_centre_" << sm << "(g) : X);\n"
+              << "    return float4(e.x, e.y, P.x, P.y);\n";
+        else
+            o << "    return e;\n";
+    } else {
+        o << "    return float4(0.0f, 0.0f, 0.0f, 0.0f);\n";
     }
     o << "}\n";
     o << "static inline float compute_" << k
@@ -271,13 +330,26 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
           << " - (lensed_deflection_ok(a) ? a : float2(1E10f, 1E10f));\n    }\n";
         plane_open = false;
     };
-    for (const auto& ob : objs) {
+    for (size_t i = 0; i < objs.size(); ++i) {
+        const ModelObject& ob = objs[i];
         std::string m = mangle(ob.name);
         if (ob.type != trigger && ob.type != OBJ_FOREGROUND) {
             if (trigger == OBJ_LENS && plane_open) close_plane();
             trigger = ob.type;
         }
-        if (ob.type == OBJ_LENS) {
+        if (fsrc >= 0 && ob.table) {
+            // lens-frame form: the deflection stays in the lens frame, a = bwd v;
+            // with bwd finite and invertible the guard on v rejects the rays
+            // the guard on a does (non-finite deflections)
+            o << "    const struct data_" << m << "* s = " << data_ptr(m, d, "data") << ";\n"
+              << "    float2 v = lensed_epl_table_vector_pre(s, pre, tab);\n"
+              << "    v = lensed_deflection_ok(v) ? v : lensed_epl_reject(s);\n";
+        } else if ((int)i == fsrc) {
+            o << "    {\n        struct data_" << m << "* g = " << data_ptr(m, d, "data") << ";\n"
+              << "        const float2 c0 = frame_map_" << m << "(g, float2(s->bwd.x, s->bwd.z));\n"
+              << "        const float2 c1 = frame_map_" << m << "(g, float2(s->bwd.y, s->bwd.w));\n"
+              << "        f += frame_profile_" << m << "(g, lensed_frame_trace(pre, c0, c1, v));\n    }\n";
+        } else if (ob.type == OBJ_LENS) {
             if (!plane_open) {
                 o << "    {\n        float2 a = 0;\n";
                 plane_open = true;

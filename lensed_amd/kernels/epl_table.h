@@ -74,13 +74,37 @@ inline float4 lensed_epl_pre_shear(const D* self, float2 r, float2 x) {
     return float4(u.x, u.y, g.x, g.y);
 }
 
-// the EPL deflection (without the shear) from the pair
+// the EPL deflection (without the shear) from the pair, in the lens frame
+// (v, with a = bwd v) and in the image frame
 template <class D>
-inline float2 lensed_epl_table_deflection_pre(const D* self, float4 pre, const float4* tab) {
+inline float2 lensed_epl_table_vector_pre(const D* self, float4 pre, const float4* tab) {
     float2 u = float2(pre.x, pre.y);
     float uu = dot(u, u);
-    float2 om = lensed_epl_omega_tab(u, tab);
-    return mv22(self->bwd, powr(uu, self->hexp) * om);
+    return powr(uu, self->hexp) * lensed_epl_omega_tab(u, tab);
+}
+
+template <class D>
+inline float2 lensed_epl_table_deflection_pre(const D* self, float4 pre, const float4* tab) {
+    return mv22(self->bwd, lensed_epl_table_vector_pre(self, pre, tab));
+}
+
+// Lens-frame form of the ray-shooting step for one source behind the lens
+// that describes itself in a frame u = T (y - c_s) (frame_map T linear):
+// with y = x - S (x - c) - bwd v the source sees
+//     u = T (x - S (x - c) - c_s) - (T bwd) v,
+// the first term formed from pixel and node parts like the pair (pre.zw),
+// the columns c0, c1 of T bwd the same for every ray of a candidate.  A ray
+// whose deflection the guard rejects moves by (1e10, 1e10), as in the
+// image-frame form: v is replaced by the v_rej with bwd v_rej = (1e10, 1e10).
+template <class D>
+inline float2 lensed_epl_reject(const D* self) {
+    const mat22 b = self->bwd;
+    const float s = 1E10f / (b.x * b.w - b.y * b.z);
+    return float2((b.w - b.y) * s, (b.x - b.z) * s);
+}
+
+inline float2 lensed_frame_trace(float4 pre, float2 c0, float2 c1, float2 v) {
+    return float2(fma(-v.y, c1.x, fma(-v.x, c0.x, pre.z)), fma(-v.y, c1.y, fma(-v.x, c0.y, pre.w)));
 }
 
 }  // namespace lcl

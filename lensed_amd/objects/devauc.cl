@@ -24,10 +24,26 @@ data
     float peak; // central surface brightness
 };
 
+// in its own frame u = tr (x - c): brightness(x) = frame_profile(frame_map(x - frame_centre))
+static float2 frame_centre(local data* this)
+{
+    return this->c;
+}
+
+static float2 frame_map(local data* this, float2 d)
+{
+    return mv22(this->tr, d);
+}
+
+static float frame_profile(local data* this, float2 u)
+{
+    float s = length(u)/this->rs;
+    return this->peak*exp(-DEVAUC_BN*sqrt(sqrt(s)));
+}
+
 static float brightness(local data* this, float2 x)
 {
-    float s = length(mv22(this->tr, x - this->c))/this->rs;
-    return this->peak*exp(-DEVAUC_BN*sqrt(sqrt(s)));
+    return frame_profile(this, frame_map(this, x - frame_centre(this)));
 }
 
 static void set(local data* this, float x, float y, float r, float mag, float q, float pa)

@@ -19,9 +19,25 @@ data
     float peak; // central surface brightness L/(2 pi rs^2 q)
 };
 
+// in its own frame u = tr (x - c): brightness(x) = frame_profile(frame_map(x - frame_centre))
+static float2 frame_centre(local data* this)
+{
+    return this->c;
+}
+
+static float2 frame_map(local data* this, float2 d)
+{
+    return mv22(this->tr, d);
+}
+
+static float frame_profile(local data* this, float2 u)
+{
+    return this->peak*exp(-length(u)/this->rs);
+}
+
 static float brightness(local data* this, float2 x)
 {
-    return this->peak*exp(-length(mv22(this->tr, x - this->c))/this->rs);
+    return frame_profile(this, frame_map(this, x - frame_centre(this)));
 }
 
 static void set(local data* this, float x, float y, float rs, float mag, float q, float pa)

@@ -18,11 +18,26 @@ data
     float lpk;  // log2 of the central surface brightness L/(2 pi sigma^2 q)
 };
 
-// peak exp(-R^2/(2 sigma^2)) as exp2(log2(peak) - |tr (x - c)|^2)
+// in its own frame u = tr (x - c): brightness(x) = frame_profile(frame_map(x - frame_centre))
+static float2 frame_centre(local data* this)
+{
+    return this->c;
+}
+
+static float2 frame_map(local data* this, float2 d)
+{
+    return mv22(this->tr, d);
+}
+
+// peak exp(-R^2/(2 sigma^2)) as exp2(log2(peak) - |u|^2)
+static float frame_profile(local data* this, float2 u)
+{
+    return exp2(fma(-u.x, u.x, fma(-u.y, u.y, this->lpk)));
+}
+
 static float brightness(local data* this, float2 x)
 {
-    float2 u = mv22(this->tr, x - this->c);
-    return exp2(fma(-u.x, u.x, fma(-u.y, u.y, this->lpk)));
+    return frame_profile(this, frame_map(this, x - frame_centre(this)));
 }
 
 static void set(local data* this, float x, float y, float sigma, float mag, float q, float pa)

@@ -23,10 +23,25 @@ data
     float hinv;   // 1/(2n)
 };
 
+// in its own frame u = tr (x - c): brightness(x) = frame_profile(frame_map(x - frame_centre))
+static float2 frame_centre(local data* this)
+{
+    return this->c;
+}
+
+static float2 frame_map(local data* this, float2 d)
+{
+    return mv22(this->tr, d);
+}
+
+static float frame_profile(local data* this, float2 u)
+{
+    return exp2(this->lnorm - 1.44269504088896340736f*exp2(this->lscale + this->hinv*log2(dot(u, u))));
+}
+
 static float brightness(local data* this, float2 x)
 {
-    float2 u = mv22(this->tr, x - this->c);
-    return exp2(this->lnorm - 1.44269504088896340736f*exp2(this->lscale + this->hinv*log2(dot(u, u))));
+    return frame_profile(this, frame_map(this, x - frame_centre(this)));
 }
 
 static void set(local data* this, float x, float y, float r, float mag, float n, float q, float pa)

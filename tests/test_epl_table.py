@@ -4,6 +4,7 @@ intervals each, FP64 nodes) restated in numpy and checked against the
 oracle's tenth-order series, including the reflection symmetries that map
 every quadrant to the first."""
 import numpy as np
+import pytest
 
 from oracle import oracle as O
 
@@ -203,3 +204,65 @@ def test_split_pair_ray_tracing_matches_oracle():
     alpha = O.deflect(objs, params, 0, x)
     scale = np.max(np.abs(alpha))
     assert np.max(np.abs(out - (x - alpha))) < 2e-5 * scale, np.max(np.abs(out - (x - alpha))) / scale
+
+
+# ------------------------------------------------------------------------
+# the table form of the generated ray code (lensed_model_source_tabulated):
+# with one framed source behind the lens, the source sees
+# u = T (x - S (x - c) - c_s) - (T bwd) v; whole rays against the oracle
+
+TABLE_HARNESS = r'''
+extern "C" void host_rays(const float* params, int w, int h, float dx, float dy, float* out) {
+    static lcl::uint data[4096];
+    lcl::set_params_0(data, params);
+    static lcl::float4 tab[LENSED_EPL_TABLE_F4];
+    build_table(params[4], params[3], tab);
+    const lcl::float4 node = lcl::epl_pre_0(data, lcl::float2(dx, dy), false);
+    for (int k = 0; k < w * h; ++k) {
+        const float cx = 1.0f + (float)(k % w), cy = 1.0f + (float)(k / w);
+        lcl::float4 b = lcl::epl_pre_0(data, lcl::float2(cx, cy), true);
+        lcl::float4 pre(b.x + node.x, b.y + node.y, b.z + node.z, b.w + node.w);
+        out[k] = lcl::compute_0(data, lcl::float2(cx + dx, cy + dy), tab, pre);
+    }
+}
+'''
+
+
+def _host_table_lib(source):
+    lib = L.load()
+    arr = (L.Object * 2)()
+    for i, (oid, name) in enumerate([("lens", "epl_plus_shear"), ("src", source)]):
+        assert lib.lensed_object_load(byref(arr[i]), oid.encode(), name.encode()) == 0
+    src = L.take_string(lib.lensed_model_source_tabulated(2, arr))
+    assert "lensed_frame_trace" in src  # the lens-frame form was generated
+    series = open(os.path.join(ROOT, "lensed_amd", "kernels", "epl_series.h")).read().replace("#pragma once", "")
+    harness = HARNESS.split("extern \"C\" void host_omega")[0]
+    src += harness.replace("@SERIES@", series) + TABLE_HARNESS
+    d = tempfile.mkdtemp(prefix="lensed_frame_")
+    cpp, so = os.path.join(d, "t.cpp"), os.path.join(d, "t.so")
+    open(cpp, "w").write(src)
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-w", "-o", so, cpp])
+    h = ctypes.CDLL(so)
+    fp = POINTER(c_float)
+    h.host_rays.argtypes = [fp, c_int, c_int, c_float, c_float, fp]
+    return h
+
+
+@pytest.mark.parametrize("source,sparams", [
+    ("gauss", [52.0, 47.5, 2.5, 0.0, 0.8, 20.0]),
+    ("sersic", [52.0, 47.5, 3.0, 0.0, 2.5, 0.7, 60.0]),
+    ("exponential", [51.0, 48.0, 3.0, 0.0, 0.6, 100.0]),
+])
+def test_lens_frame_rays_match_oracle(source, sparams):
+    """One ray per pixel at node offset (dx, dy): the generated table form
+    (pixel and node parts, the source's frame, T bwd applied to the
+    lens-frame deflection) against the FP64 oracle's ray-shoot."""
+    h = _host_table_lib(source)
+    objs = [("lens", "epl_plus_shear"), ("src", source)]
+    params = np.array([50.3, 49.7, 12.0, 1.2, 0.65, 35.0, 0.03, -0.02] + sparams)
+    fp = POINTER(c_float)
+    for dx, dy in [(0.0, 0.0), (0.31, -0.22)]:
+        out = np.zeros((100, 100), np.float32)
+        h.host_rays(params.astype(np.float32).ctypes.data_as(fp), 100, 100, dx, dy, out.ctypes.data_as(fp))
+        ref, _ = O.render(objs, params, (100, 100), [[dx, dy]], [[1.0, 0.0]])
+        assert np.max(np.abs(out - ref)) < 2e-5 * np.max(ref), (dx, dy, np.max(np.abs(out - ref)) / np.max(ref))
