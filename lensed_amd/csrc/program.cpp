@@ -338,12 +338,11 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
             trigger = ob.type;
         }
         if (fsrc >= 0 && ob.table) {
-            // lens-frame form: the deflection stays in the lens frame, a = bwd v;
-            // with bwd finite and invertible the guard on v rejects the rays
-            // the guard on a does (non-finite deflections)
+            // lens-frame form: the deflection stays in the lens frame, a = bwd v,
+            // and the guard clamps v (lensed_clamp_deflection)
             o << "    const struct data_" << m << "* s = " << data_ptr(m, d, "data") << ";\n"
               << "    float2 v = lensed_epl_table_vector_pre(s, pre, tab);\n"
-              << "    v = lensed_deflection_ok(v) ? v : lensed_epl_reject(s);\n";
+              << "    v = lensed_clamp_deflection(v);\n";
         } else if ((int)i == fsrc) {
             o << "    {\n        struct data_" << m << "* g = " << data_ptr(m, d, "data") << ";\n"
               << "        const float2 c0 = frame_map_" << m << "(g, float2(s->bwd.x, s->bwd.z));\n"

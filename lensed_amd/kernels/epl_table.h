@@ -93,14 +93,15 @@ inline float2 lensed_epl_table_deflection_pre(const D* self, float4 pre, const f
 // with y = x - S (x - c) - bwd v the source sees
 //     u = T (x - S (x - c) - c_s) - (T bwd) v,
 // the first term formed from pixel and node parts like the pair (pre.zw),
-// the columns c0, c1 of T bwd the same for every ray of a candidate.  A ray
-// whose deflection the guard rejects moves by (1e10, 1e10), as in the
-// image-frame form: v is replaced by the v_rej with bwd v_rej = (1e10, 1e10).
-template <class D>
-inline float2 lensed_epl_reject(const D* self) {
-    const mat22 b = self->bwd;
-    const float s = 1E10f / (b.x * b.w - b.y * b.z);
-    return float2((b.w - b.y) * s, (b.x - b.z) * s);
+// the columns c0, c1 of T bwd the same for every ray of a candidate.
+
+// The guard of the lens-frame form: each component of v is clamped to
+// [-1e10, 1e10], a NaN going to -1e10 (fmax returns its other operand).  A
+// ray whose deflection is not finite, or more than ~1e10 |bwd| pixels, so
+// ends ~1e10 pixels out, where every framed source's brightness is 0 in
+// FP32 -- as at the 1e10 displacement the image-frame form gives it.
+inline float2 lensed_clamp_deflection(float2 v) {
+    return float2(fminf(fmaxf(v.x, -1E10f), 1E10f), fminf(fmaxf(v.y, -1E10f), 1E10f));
 }
 
 inline float2 lensed_frame_trace(float4 pre, float2 c0, float2 c1, float2 v) {

@@ -266,3 +266,21 @@ def test_lens_frame_rays_match_oracle(source, sparams):
         h.host_rays(params.astype(np.float32).ctypes.data_as(fp), 100, 100, dx, dy, out.ctypes.data_as(fp))
         ref, _ = O.render(objs, params, (100, 100), [[dx, dy]], [[1.0, 0.0]])
         assert np.max(np.abs(out - ref)) < 2e-5 * np.max(ref), (dx, dy, np.max(np.abs(out - ref)) / np.max(ref))
+
+
+@pytest.mark.parametrize("t", [0.7, 1.3])
+def test_lens_frame_ray_through_the_lens_centre(t):
+    """A ray exactly at the lens centre (u = 0): the deflection is 0 for
+    t < 1 and not finite for t > 1; the guard must send the latter out of
+    the source (brightness 0, as the reference's rejected ray), not return
+    a NaN."""
+    h = _host_table_lib("gauss")
+    objs = [("lens", "epl_plus_shear"), ("src", "gauss")]
+    params = np.array([50.0, 50.0, 12.0, t, 0.65, 35.0, 0.03, -0.02, 50.5, 50.2, 2.5, 0.0, 0.8, 20.0])
+    fp = POINTER(c_float)
+    out = np.zeros((100, 100), np.float32)
+    h.host_rays(params.astype(np.float32).ctypes.data_as(fp), 100, 100, 0.0, 0.0, out.ctypes.data_as(fp))
+    ref, _ = O.render(objs, params, (100, 100), [[0.0, 0.0]], [[1.0, 0.0]])
+    assert np.all(np.isfinite(out))
+    assert out[49, 49] == pytest.approx(ref[49, 49], abs=1e-6 * np.max(ref))  # pixel (50, 50)
+    assert np.max(np.abs(out - ref)) < 2e-5 * np.max(ref)
