@@ -6,7 +6,7 @@
 #define LENSED_RENDER_BS 256  // threads (= pixels) per render work item
 #define LENSED_CONV_TX 64     // conv tile width  (threads along x)
 #define LENSED_CONV_TY 4      // threads along y
-#define LENSED_CONV_RY 8      // output rows per thread, large launches
+#define LENSED_CONV_RY 8      // output rows per thread, large launches (a multiple of 8)
 #define LENSED_CONV_RY_SMALL 2  // output rows per thread, launches too small to fill the GPU
 // chi^2 partial sums are formed over fixed 8-row x 64-column blocks, so the
 // short tile (4 x 2 rows) is exactly one block and the tall one (4 x 8) four

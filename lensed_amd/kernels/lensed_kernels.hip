@@ -456,28 +456,32 @@ __device__ inline void lensed_conv_item(const lensed_desc& d, int cand, int t,
             if (chi2_out) chi2_out[d.model_off + (long long)cand * d.npix + kk] = chi[r];
         }
     }
-    // canonical 8-row blocks: a thread of the tall tile owns one block
-    // column; the short tile's 4 threads pass their rows through LDS
-    float s = 0.0f, c = 0.0f;
-    int block_row;
-    if (RY == 8) {
+    // canonical 8-row blocks: a thread of the tall tile owns a column of
+    // RY / 8 blocks; the short tile's 4 threads pass their rows through LDS
+    double* part = partials + d.partial_off + (long long)cand * d.npart + t % d.tiles_x;
+    if (RY % 8 == 0) {
 #pragma unroll
-        for (int r = 0; r < RY; ++r) lensed_two_sum_add(s, c, chi[r]);
-        block_row = (ty0 + ly * RY) / 8;
+        for (int g = 0; g < RY / 8; ++g) {
+            float s = 0.0f, c = 0.0f;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) lensed_two_sum_add(s, c, chi[8 * g + r]);
+            const double blk = lensed_wave_csum(s, c);
+            const int block_row = (ty0 + ly * RY) / 8 + g;
+            if (lx == 0 && block_row * 8 < H) part[(long long)block_row * d.tiles_x] = blk;
+        }
     } else {
+        float s = 0.0f, c = 0.0f;
 #pragma unroll
         for (int r = 0; r < RY; ++r) rows8[(ly * RY + r) * TX + lx] = chi[r];
         __syncthreads();
-        if (ly == 0)
+        if (ly == 0) {
             for (int r = 0; r < 8; ++r) lensed_two_sum_add(s, c, rows8[r * TX + lx]);
-        block_row = ty0 / 8;
+            const double blk = lensed_wave_csum(s, c);
+            const int block_row = ty0 / 8;
+            if (lx == 0 && block_row * 8 < H) part[(long long)block_row * d.tiles_x] = blk;
+        }
+        __syncthreads();  // rows8 is reused by the next item
     }
-    if (RY == 8 || ly == 0) {
-        double blk = lensed_wave_csum(s, c);
-        if (lx == 0 && block_row * 8 < H)
-            partials[d.partial_off + (long long)cand * d.npart + (long long)block_row * d.tiles_x + t % d.tiles_x] = blk;
-    }
-    if (RY != 8) __syncthreads();  // rows8 is reused by the next item
     (void)red;
 }
 
