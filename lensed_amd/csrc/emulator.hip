@@ -467,15 +467,17 @@ __global__ __launch_bounds__(256) void k_emu_assemble(const lensed_desc* __restr
     int half = rem / LENSED_EPL_ROWS, i = rem - half * LENSED_EPL_ROWS;
     long long cb = d.table_off / LENSED_EPL_TABLE_F4 + cand;
     float4 a = nodes[cb * kNodes + half * (LENSED_EPL_NT + 1) + i];
-    float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4 + 2 * rem;
+    float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
+    float4& ox = out[LENSED_EPL_AT(half, 0, i)];
+    float4& oy = out[LENSED_EPL_AT(half, 1, i)];
     if (i == LENSED_EPL_NT) {  // the end point tau = 1 as a constant row
-        out[0] = float4(a.x, 0.0f, 0.0f, 0.0f);
-        out[1] = float4(a.y, 0.0f, 0.0f, 0.0f);
+        ox = float4(a.x, 0.0f, 0.0f, 0.0f);
+        oy = float4(a.y, 0.0f, 0.0f, 0.0f);
         return;
     }
     float4 b = nodes[cb * kNodes + half * (LENSED_EPL_NT + 1) + i + 1];
-    out[0] = lensed_hermite(a.x, a.z, b.x, b.z);
-    out[1] = lensed_hermite(a.y, a.w, b.y, b.w);
+    ox = lensed_hermite(a.x, a.z, b.x, b.z);
+    oy = lensed_hermite(a.y, a.w, b.y, b.w);
 }
 
 // held-out check, one wave per candidate: the table against the FP64 series
@@ -500,7 +502,7 @@ __global__ __launch_bounds__(256) void k_emu_check(const lensed_desc* __restrict
         int half = m / LENSED_EPL_NT, i = m - half * LENSED_EPL_NT;
         double ox, oy, dx, dy;
         lensed_epl_node(q, t, half, (i + 0.5) / LENSED_EPL_NT, ox, oy, dx, dy);
-        float4 cx = tab[2 * (half * LENSED_EPL_ROWS + i)], cy = tab[2 * (half * LENSED_EPL_ROWS + i) + 1];
+        float4 cx = tab[LENSED_EPL_AT(half, 0, i)], cy = tab[LENSED_EPL_AT(half, 1, i)];
         float f = 0.5f;
         float tx = cx.x + f * (cx.y + f * (cx.z + f * cx.w));
         float ty = cy.x + f * (cy.y + f * (cy.z + f * cy.w));

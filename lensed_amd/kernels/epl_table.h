@@ -12,7 +12,8 @@
 // coordinate f in [0, 1) for each component, as two float4 (a0, a1, a2, a3):
 //     O(f) = a0 + f (a1 + f (a2 + f a3)),
 // and one more row per half holds the value at tau = 1 (a1 = a2 = a3 = 0),
-// so the row index floor(tau NT) needs no clamp.
+// so the row index floor(tau NT) needs no clamp.  Rows are stored per
+// (half, component), LENSED_EPL_AT in desc.h.
 // The table of a candidate is built from the series (or by the emulator)
 // and staged in LDS by the render kernel.
 //
@@ -49,9 +50,9 @@ inline float2 lensed_epl_omega_tab(float2 u, const float4* tab) {
     float s = (lo / hi) * (float)LENSED_EPL_NT;
     int i = (int)s;
     float f = lensed_fract(s);
-    const float4* row = (av > au ? tab + 2 * LENSED_EPL_ROWS : tab) + 2 * i;
+    const float4* row = tab + LENSED_EPL_AT(av > au ? 1 : 0, 0, i);
     const float4 cx = row[0];
-    const float4 cy = row[1];
+    const float4 cy = row[LENSED_EPL_ROWS];
     float ox = cx.x + f * (cx.y + f * (cx.z + f * cx.w));
     float oy = cy.x + f * (cy.y + f * (cy.z + f * cy.w));
     return float2(copysign(ox, u.x), copysign(oy, u.y));

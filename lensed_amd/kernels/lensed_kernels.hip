@@ -188,17 +188,19 @@ extern "C" __global__ __launch_bounds__(256) void lensed_epl_tables(
     const float* p = params + d.param_off + (long long)cand * d.npar;
     double q = p[d.epl_q], t = p[d.epl_t];
     double x0, y0, dx0, dy0, x1, y1, dx1, dy1;
-    float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4 + 2 * rem;
+    float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
+    float4& ox = out[LENSED_EPL_AT(half, 0, i)];
+    float4& oy = out[LENSED_EPL_AT(half, 1, i)];
     if (i == LENSED_EPL_NT) {  // the end point tau = 1 as a constant row
         lensed_epl_node(q, t, half, 1.0, x1, y1, dx1, dy1);
-        out[0] = float4((float)x1, 0.0f, 0.0f, 0.0f);
-        out[1] = float4((float)y1, 0.0f, 0.0f, 0.0f);
+        ox = float4((float)x1, 0.0f, 0.0f, 0.0f);
+        oy = float4((float)y1, 0.0f, 0.0f, 0.0f);
         return;
     }
     lensed_epl_node(q, t, half, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
     lensed_epl_node(q, t, half, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
-    out[0] = lensed_hermite(x0, dx0, x1, dx1);
-    out[1] = lensed_hermite(y0, dy0, y1, dy1);
+    ox = lensed_hermite(x0, dx0, x1, dx1);
+    oy = lensed_hermite(y0, dy0, y1, dy1);
 }
 
 // --------------------------------------------------------------- conv+chi2

@@ -106,17 +106,18 @@ static void build_table(double q, double t, lcl::float4* tab) {
     for (int half = 0; half < 2; ++half)
         for (int i = 0; i <= LENSED_EPL_NT; ++i) {
             double x0, y0, dx0, dy0, x1, y1, dx1, dy1;
-            lcl::float4* out = tab + 2 * (half * LENSED_EPL_ROWS + i);
+            lcl::float4& ox = tab[LENSED_EPL_AT(half, 0, i)];
+            lcl::float4& oy = tab[LENSED_EPL_AT(half, 1, i)];
             if (i == LENSED_EPL_NT) {
                 hostseries::lensed_epl_node(q, t, half, 1.0, x1, y1, dx1, dy1);
-                out[0] = lcl::float4((float)x1, 0.f, 0.f, 0.f);
-                out[1] = lcl::float4((float)y1, 0.f, 0.f, 0.f);
+                ox = lcl::float4((float)x1, 0.f, 0.f, 0.f);
+                oy = lcl::float4((float)y1, 0.f, 0.f, 0.f);
                 continue;
             }
             hostseries::lensed_epl_node(q, t, half, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
             hostseries::lensed_epl_node(q, t, half, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
-            out[0] = hostseries::lensed_hermite(x0, dx0, x1, dx1);
-            out[1] = hostseries::lensed_hermite(y0, dy0, y1, dy1);
+            ox = hostseries::lensed_hermite(x0, dx0, x1, dx1);
+            oy = hostseries::lensed_hermite(y0, dy0, y1, dy1);
         }
 }
 extern "C" void host_omega(double q, double t, const float* u, int n, float* out) {
