@@ -341,13 +341,12 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
             // lens-frame form: the deflection stays in the lens frame, a = bwd v,
             // and the guard clamps v (lensed_clamp_deflection)
             o << "    const struct data_" << m << "* s = " << data_ptr(m, d, "data") << ";\n"
-              << "    float2 v = lensed_epl_table_vector_pre(s, pre, tab);\n"
-              << "    v = lensed_clamp_deflection(v);\n";
+              << "    const float2 v = lensed_clamp_deflection(lensed_epl_table_vector_pre(s, pre, tab));\n";
         } else if ((int)i == fsrc) {
             o << "    {\n        struct data_" << m << "* g = " << data_ptr(m, d, "data") << ";\n"
-              << "        const float2 c0 = frame_map_" << m << "(g, float2(s->bwd.x, s->bwd.z));\n"
-              << "        const float2 c1 = frame_map_" << m << "(g, float2(s->bwd.y, s->bwd.w));\n"
-              << "        f += frame_profile_" << m << "(g, lensed_frame_trace(pre, c0, c1, v));\n    }\n";
+              << "        const float2 n0 = frame_map_" << m << "(g, float2(-s->bwd.x, -s->bwd.z));\n"
+              << "        const float2 n1 = frame_map_" << m << "(g, float2(-s->bwd.y, -s->bwd.w));\n"
+              << "        f += frame_profile_" << m << "(g, lensed_frame_trace(pre, n0, n1, v));\n    }\n";
         } else if (ob.type == OBJ_LENS) {
             if (!plane_open) {
                 o << "    {\n        float2 a = 0;\n";

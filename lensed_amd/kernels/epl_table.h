@@ -47,10 +47,12 @@ inline float2 lensed_epl_omega_tab(float2 u, const float4* tab) {
     float au = fabs(u.x), av = fabs(u.y);
     // the floor keeps tau = 0 (not 0/0) at u = 0
     float lo = fmin(au, av), hi = fmax(fmax(au, av), 1e-30f);
-    float s = (lo / hi) * (float)LENSED_EPL_NT;
+    // the row index with the octant's offset added in float: s < 512, so its
+    // fraction keeps 15 bits (the interpolation error that adds is < 1e-7)
+    float s = fmaf(lo / hi, (float)LENSED_EPL_NT, av > au ? (float)LENSED_EPL_AT(1, 0, 0) : 0.0f);
     int i = (int)s;
     float f = lensed_fract(s);
-    const float4* row = tab + LENSED_EPL_AT(av > au ? 1 : 0, 0, i);
+    const float4* row = tab + i;
     const float4 cx = row[0];
     const float4 cy = row[LENSED_EPL_ROWS];
     float ox = cx.x + f * (cx.y + f * (cx.z + f * cx.w));
@@ -97,16 +99,20 @@ inline float2 lensed_epl_table_deflection_pre(const D* self, float4 pre, const f
 // the columns c0, c1 of T bwd the same for every ray of a candidate.
 
 // The guard of the lens-frame form: each component of v is clamped to
-// [-1e10, 1e10], a NaN going to -1e10 (fmax returns its other operand).  A
-// ray whose deflection is not finite, or more than ~1e10 |bwd| pixels, so
-// ends ~1e10 pixels out, where every framed source's brightness is 0 in
-// FP32 -- as at the 1e10 displacement the image-frame form gives it.
+// [-1e10, 1e10], a NaN going to -1e10 (fmax returns its other operand).
+// The only non-finite v of finite parameters is at u = 0 with t >= 1 (where
+// the image-frame guard rejects the ray); such a ray, and any deflected by
+// more than ~1e10 |bwd| pixels, ends ~1e10 pixels out, where every framed
+// source's brightness is 0 in FP32 -- as at the 1e10 displacement the
+// image-frame form gives a rejected ray.  At u = 0 with t < 1, v = 0
+// exactly, as in the reference.
 inline float2 lensed_clamp_deflection(float2 v) {
     return float2(fminf(fmaxf(v.x, -1E10f), 1E10f), fminf(fmaxf(v.y, -1E10f), 1E10f));
 }
 
-inline float2 lensed_frame_trace(float4 pre, float2 c0, float2 c1, float2 v) {
-    return float2(fma(-v.y, c1.x, fma(-v.x, c0.x, pre.z)), fma(-v.y, c1.y, fma(-v.x, c0.y, pre.w)));
+// (n0, n1 = -c0, -c1: the negation stays with the per-candidate columns)
+inline float2 lensed_frame_trace(float4 pre, float2 n0, float2 n1, float2 v) {
+    return float2(fma(v.y, n1.x, fma(v.x, n0.x, pre.z)), fma(v.y, n1.y, fma(v.x, n0.y, pre.w)));
 }
 
 }  // namespace lcl

@@ -97,12 +97,10 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
 #pragma unroll
     for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = i < d.dwords ? __builtin_amdgcn_readfirstlane(gb[i]) : 0u;
     const float4* q = quad + d.quad_off;
-    const lcl::float4* tab = nullptr;
     if (d.table_mode) {  // the candidate's EPL angular table, staged in LDS
         const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
         for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
         __syncthreads();
-        tab = (const lcl::float4*)tab_lds;
     }
     int k = tile * LENSED_RENDER_BS + (int)threadIdx.x;
     if (k < d.npix) {
@@ -121,14 +119,15 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
                 const float4 node = q[n];
                 const float4 np = np_c[n];
                 const lcl::float4 pre(base.x + np.x, base.y + np.y, base.z + np.z, base.w + np.w);
-                float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y), tab, pre);
+                float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y),
+                                                (const lcl::float4*)tab_lds, pre);
                 f += node.z * v;
                 if (ERR) e += node.w * v;
             }
         } else {
             for (int n = 0; n < d.nq; ++n) {
                 const float4 node = q[n];
-                float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y), tab,
+                float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y), nullptr,
                                                 lcl::float4(0.0f, 0.0f, 0.0f, 0.0f));
                 f += node.z * v;
                 if (ERR) e += node.w * v;
