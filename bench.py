@@ -39,6 +39,12 @@ FLOP_CHI2 = 4 * 65536
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X published FP32 (packed / matrix) peak
 PEAK_FP32_NONPACKED = 78.6
 PEAK_HBM = 8000.0  # GB/s
+# The render kernel is bound by VALU issue: one wave64 VALU instruction per
+# CU per cycle (256 CUs at 2.4 GHz).  Its C3 ray (tabulated EPL + shear in the
+# lens-frame form, Gaussian source) is 35 VALU instructions in the gfx950 ISA
+# of the generated program (rocprofv3 SQ_INSTS_VALU per launch agrees)
+RAY_VALU = 35
+VALU_ISSUE_PER_S = 256 * 2.4e9  # wave instructions per second
 # HBM bytes per 4096-candidate render launch, measured with rocprofv3 --pmc:
 # FETCH_SIZE 140300 KiB (x 2, the factor a 1 GiB copy calibrates) in and
 # WRITE_SIZE 1048576 KiB (x 1) out, i.e. the 4096 model images
@@ -343,7 +349,12 @@ def main():
                      "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED, 4),
                      "traffic": int(TRAFFIC_RENDER_4096 * K / 4096), "traffic_unit": "bytes per launch (rocprofv3 --pmc)",
                      "algorithmic_flop_per_launch": FLOP_RENDER * K,
-                     "avg_launch_ms": round(kt["render"], 4)},
+                     "avg_launch_ms": round(kt["render"], 4),
+                     "valu_instr_per_ray": RAY_VALU,
+                     "valu_issue_frac": round(RAY_VALU * 65536 * 49 * K / 64 / VALU_ISSUE_PER_S / (kt["render"] / 1e3), 4),
+                     "note": "achieved counts the reference formulas (165 flop per ray, the tenth-order EPL series); "
+                             "the table path does that work in 35 VALU instructions per ray, so frac can exceed 1: "
+                             "valu_issue_frac is the kernel's position against its actual bound"},
         "stages_ms_per_launch": {k: round(v, 4) for k, v in kt.items()},
         "stages_note": "set_params includes the emulator, held-out check and fallback table kernels",
         "stages_ms_per_launch_exact": {k: round(v, 4) for k, v in kt_exact.items()},

@@ -407,8 +407,9 @@ __global__ __launch_bounds__(256) void k_mlp_raw(int n, const float* __restrict_
     }
 }
 
-// table nodes: one sample per (candidate, octant, node j = 0..NT); the node
-// value Omega and its f-derivative, from the network
+// table nodes: one sample per (candidate, node j = 0..NT); the network gives
+// Omega and its derivative at psi(p_j), turned into the tabulated factor
+// W = g^h Omega and its f-derivative as lensed_epl_node (epl_series.h) does
 template <int WHICH>
 __device__ inline int find_sys_t(const lensed_desc* __restrict__ sys, int nsys, int item) {
     int lo = 0, hi = nsys - 1;
@@ -420,7 +421,7 @@ __device__ inline int find_sys_t(const lensed_desc* __restrict__ sys, int nsys, 
     return lo;
 }
 
-constexpr int kNodes = 2 * (LENSED_EPL_NT + 1);  // per candidate
+constexpr int kNodes = LENSED_EPL_ROWS;  // per candidate
 
 __global__ __launch_bounds__(256) void k_emu_nodes(const lensed_desc* __restrict__ sys, int nsys, int nitems,
                                                    const float* __restrict__ params, const float* __restrict__ wf,
@@ -434,13 +435,12 @@ __global__ __launch_bounds__(256) void k_emu_nodes(const lensed_desc* __restrict
     const int local = gc - d.item_e;
     const int cand = local / kNodes;
     const int rem = local - cand * kNodes;
-    const int half = rem / (LENSED_EPL_NT + 1), j = rem - half * (LENSED_EPL_NT + 1);
     const float* p = params + d.param_off + (long long)cand * d.npar;
     const float q = p[d.epl_q], t = p[d.epl_t];
-    const float tau = (float)j / LENSED_EPL_NT;
-    const float at = atanf(tau);
-    const float psi = half ? 1.57079632679489661923f - at : at;
-    const float dpsi = (half ? -1.0f : 1.0f) / (1.0f + tau * tau) / LENSED_EPL_NT;  // d psi / d f
+    const float pj = (float)rem / LENSED_EPL_NT;
+    const float gj = pj * pj + (1.0f - pj) * (1.0f - pj), h = 0.5f * (1.0f - t);
+    const float psi = atan2f(pj, 1.0f - pj);
+    const float dpsi = 1.0f / gj / LENSED_EPL_NT;  // d psi / d f
     float sn, cs;
     sincosf(psi, &sn, &cs);
     float x[4] = {(q - kQ0) / kQS, (t - kT0) / kTS, cs, sn};
@@ -449,7 +449,10 @@ __global__ __launch_bounds__(256) void k_emu_nodes(const lensed_desc* __restrict
     mlp_wave(wf, wp, wp + 4 * 16 * 64, x, dx, y, dy, lane);
     if (lane < 32 && g < nitems) {
         long long cb = d.table_off / LENSED_EPL_TABLE_F4 + cand;
-        nodes[cb * kNodes + rem] = make_float4(cs + y[0], sn + y[1], dx[2] + dy[0], dx[3] + dy[1]);
+        const float gh = powf(gj, h), lg = h * (4.0f * pj - 2.0f) / gj / LENSED_EPL_NT;  // d log(g^h) / df
+        const float ox = cs + y[0], oy = sn + y[1];
+        nodes[cb * kNodes + rem] =
+            make_float4(gh * ox, gh * oy, gh * (lg * ox + dx[2] + dy[0]), gh * (lg * oy + dx[3] + dy[1]));
     }
 }
 
@@ -462,20 +465,19 @@ __global__ __launch_bounds__(256) void k_emu_assemble(const lensed_desc* __restr
     int s = find_sys_t<0>(sys, nsys, item);
     const lensed_desc& d = sys[s];
     int local = item - d.item_t;
-    int cand = local / (2 * LENSED_EPL_ROWS);
-    int rem = local - cand * 2 * LENSED_EPL_ROWS;
-    int half = rem / LENSED_EPL_ROWS, i = rem - half * LENSED_EPL_ROWS;
+    int cand = local / LENSED_EPL_ROWS;
+    int i = local - cand * LENSED_EPL_ROWS;
     long long cb = d.table_off / LENSED_EPL_TABLE_F4 + cand;
-    float4 a = nodes[cb * kNodes + half * (LENSED_EPL_NT + 1) + i];
+    float4 a = nodes[cb * kNodes + i];
     float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
-    float4& ox = out[LENSED_EPL_AT(half, 0, i)];
-    float4& oy = out[LENSED_EPL_AT(half, 1, i)];
-    if (i == LENSED_EPL_NT) {  // the end point tau = 1 as a constant row
+    float4& ox = out[LENSED_EPL_AT(0, i)];
+    float4& oy = out[LENSED_EPL_AT(1, i)];
+    if (i == LENSED_EPL_NT) {  // the end point p = 1 as a constant row
         ox = float4(a.x, 0.0f, 0.0f, 0.0f);
         oy = float4(a.y, 0.0f, 0.0f, 0.0f);
         return;
     }
-    float4 b = nodes[cb * kNodes + half * (LENSED_EPL_NT + 1) + i + 1];
+    float4 b = nodes[cb * kNodes + i + 1];
     ox = lensed_hermite(a.x, a.z, b.x, b.z);
     oy = lensed_hermite(a.y, a.w, b.y, b.w);
 }
@@ -498,11 +500,10 @@ __global__ __launch_bounds__(256) void k_emu_check(const lensed_desc* __restrict
     double q = p[d.epl_q], t = p[d.epl_t];
     const float4* tab = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
     float err = 0.0f;
-    for (int m = lane; m < 2 * LENSED_EPL_NT; m += 64) {
-        int half = m / LENSED_EPL_NT, i = m - half * LENSED_EPL_NT;
+    for (int i = lane; i < LENSED_EPL_NT; i += 64) {
         double ox, oy, dx, dy;
-        lensed_epl_node(q, t, half, (i + 0.5) / LENSED_EPL_NT, ox, oy, dx, dy);
-        float4 cx = tab[LENSED_EPL_AT(half, 0, i)], cy = tab[LENSED_EPL_AT(half, 1, i)];
+        lensed_epl_node(q, t, (i + 0.5) / LENSED_EPL_NT, ox, oy, dx, dy);
+        float4 cx = tab[LENSED_EPL_AT(0, i)], cy = tab[LENSED_EPL_AT(1, i)];
         float f = 0.5f;
         float tx = cx.x + f * (cx.y + f * (cx.z + f * cx.w));
         float ty = cy.x + f * (cy.y + f * (cy.z + f * cy.w));

@@ -12,13 +12,13 @@
 // short tile (4 x 2 rows) is exactly one block and the tall one (4 x 8) four
 #define LENSED_CONV_BS (LENSED_CONV_TX * LENSED_CONV_TY)
 #define LENSED_PSF_MAX 63     // largest PSF side the conv tile is sized for
-#define LENSED_EPL_NT 128     // EPL angular table: intervals per octant
-#define LENSED_EPL_ROWS (LENSED_EPL_NT + 1)  // rows per octant: the intervals and the end point tau = 1
-#define LENSED_EPL_TABLE_F4 (2 * LENSED_EPL_ROWS * 2)  // float4 per candidate (2 octants x 2 components)
-// float4 index of row i of component comp (0: x, 1: y) in octant half: each
-// component's rows are contiguous (16-byte stride), so lanes reading nearby
-// rows hit different LDS banks
-#define LENSED_EPL_AT(half, comp, i) ((((half) * 2 + (comp)) * LENSED_EPL_ROWS) + (i))
+#define LENSED_EPL_NT 256     // EPL angular table: intervals of the diamond angle p in [0, 1]
+#define LENSED_EPL_ROWS (LENSED_EPL_NT + 1)  // rows per component: the intervals and the end point p = 1
+#define LENSED_EPL_TABLE_F4 (2 * LENSED_EPL_ROWS)  // float4 per candidate (2 components)
+// float4 index of row i of component comp (0: x, 1: y): each component's
+// rows are contiguous (16-byte stride), so lanes reading nearby rows hit
+// different LDS banks
+#define LENSED_EPL_AT(comp, i) ((comp) * LENSED_EPL_ROWS + (i))
 
 struct lensed_desc {
     int width, height, npix, model;  // image size, pixels, model index

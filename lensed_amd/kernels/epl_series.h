@@ -1,7 +1,7 @@
 // The EPL angular factor in FP64: Omega(psi) = sum_{k=0..10} C_k e^{i(2k+1) psi}
 // with C_0 = 1, C_k = C_{k-1} (-f (2k - T)/(2k + T)), f = (1-q)/(1+q), T = 2-t
 // (the tenth-order series of objects/epl.cl), and its psi derivative.
-// Table nodes are taken in the octant variable tau of epl_table.h.
+// Table nodes are taken in the diamond angle p of epl_table.h.
 #pragma once
 
 __device__ inline void lensed_epl_omega(double q, double t, double psi, double& ox, double& oy, double& dx,
@@ -21,18 +21,23 @@ __device__ inline void lensed_epl_omega(double q, double t, double psi, double& 
     }
 }
 
-// octant `half` (0: psi = atan tau, 1: psi = pi/2 - atan tau); the
-// derivative is per unit of the interval coordinate f = tau * NT - i
-__device__ inline void lensed_epl_node(double q, double t, int half, double tau, double& ox, double& oy,
-                                       double& dx, double& dy)
+// The tabulated factor at diamond angle p = sin / (cos + sin) of psi in the
+// first quadrant: W(p) = g(p)^h Omega(psi(p)) with g = p^2 + (1-p)^2
+// (|u|^2 = (|u_x| + |u_y|)^2 g) and h = (1-t)/2, so that the deflection's
+// |u|^(1-t) Omega is (|u_x| + |u_y|)^(1-t) W.  psi = atan2(p, 1-p), dpsi/dp
+// = 1/g; the derivative is per unit of the interval coordinate f = p NT - i.
+__device__ inline void lensed_epl_node(double q, double t, double p, double& wx, double& wy, double& dx,
+                                       double& dy)
 {
-    double a = atan(tau);
-    double psi = half ? 1.57079632679489661923 - a : a;
-    double gx, gy;
-    lensed_epl_omega(q, t, psi, ox, oy, gx, gy);
-    double dpsi = (half ? -1.0 : 1.0) / (1 + tau * tau) / LENSED_EPL_NT;
-    dx = gx * dpsi;
-    dy = gy * dpsi;
+    const double g = p * p + (1 - p) * (1 - p), h = 0.5 * (1 - t);
+    const double gh = pow(g, h);
+    double ox, oy, gx, gy;
+    lensed_epl_omega(q, t, atan2(p, 1 - p), ox, oy, gx, gy);
+    const double lg = h * (4 * p - 2) / g;  // d log(g^h) / dp
+    wx = gh * ox;
+    wy = gh * oy;
+    dx = gh * (lg * ox + gx / g) / LENSED_EPL_NT;
+    dy = gh * (lg * oy + gy / g) / LENSED_EPL_NT;
 }
 
 // cubic Hermite coefficients (a0, a1, a2, a3) of one interval from the

@@ -5,17 +5,19 @@
 // Omega has real series coefficients, so Omega_x is odd in u_x and even in
 // u_y and Omega_y the reverse: Omega(u) = (sgn u_x Ox(psi), sgn u_y Oy(psi))
 // with psi the first-quadrant angle of (|u_x|, |u_y|).  The quadrant is
-// covered by two octants in the variable tau = min(|u_x|, |u_y|) /
-// max(|u_x|, |u_y|) in [0, 1]: half 0 is psi = atan(tau), half 1 is
-// psi = pi/2 - atan(tau).  Each half has LENSED_EPL_NT intervals; per
-// interval the table holds the cubic Hermite polynomial in the local
-// coordinate f in [0, 1) for each component, as two float4 (a0, a1, a2, a3):
-//     O(f) = a0 + f (a1 + f (a2 + f a3)),
-// and one more row per half holds the value at tau = 1 (a1 = a2 = a3 = 0),
-// so the row index floor(tau NT) needs no clamp.  Rows are stored per
-// (half, component), LENSED_EPL_AT in desc.h.
-// The table of a candidate is built from the series (or by the emulator)
-// and staged in LDS by the render kernel.
+// indexed by the diamond angle p = |u_y| / (|u_x| + |u_y|) in [0, 1] (one
+// division, no octant choice), and with s = |u_x| + |u_y| the deflection's
+// |u|^(1-t) Omega = s^(1-t) W(p), W = g^h Omega, g = p^2 + (1-p)^2,
+// h = (1-t)/2: the radial power needs log2(s) only, the rest is in the
+// table.  LENSED_EPL_NT intervals; per interval the table holds the cubic
+// Hermite polynomial in the local coordinate f in [0, 1) for each
+// component, as float4 (a0, a1, a2, a3):
+//     W(f) = a0 + f (a1 + f (a2 + f a3)),
+// and one more row holds the value at p = 1 (a1 = a2 = a3 = 0), so the row
+// index floor(p NT) needs no clamp.  Rows are stored per component,
+// LENSED_EPL_AT in desc.h.  The table of a candidate is built from the
+// series (or by the emulator, epl_series.h / emulator.hip) and staged in
+// LDS by the render kernel.
 //
 // The lens-frame position u = F (x - c) and the ray position with the
 // lens's linear (external shear) deflection already taken off, x - S (x - c),
@@ -27,8 +29,8 @@
 // ray.
 #pragma once
 
-// LENSED_EPL_NT (intervals per octant), LENSED_EPL_ROWS and
-// LENSED_EPL_TABLE_F4 (float4 per candidate) come from desc.h.
+// LENSED_EPL_NT (intervals), LENSED_EPL_ROWS and LENSED_EPL_TABLE_F4
+// (float4 per candidate) come from desc.h.
 
 #ifdef LCL_GPU
 #pragma clang force_cuda_host_device begin
@@ -43,21 +45,18 @@ inline float lensed_fract(float s) {
 #endif
 }
 
-inline float2 lensed_epl_omega_tab(float2 u, const float4* tab) {
-    float au = fabs(u.x), av = fabs(u.y);
-    // the floor keeps tau = 0 (not 0/0) at u = 0
-    float lo = fmin(au, av), hi = fmax(fmax(au, av), 1e-30f);
-    // the row index with the octant's offset added in float: s < 512, so its
-    // fraction keeps 15 bits (the interpolation error that adds is < 1e-7)
-    float s = fmaf(lo / hi, (float)LENSED_EPL_NT, av > au ? (float)LENSED_EPL_AT(1, 0, 0) : 0.0f);
-    int i = (int)s;
-    float f = lensed_fract(s);
-    const float4* row = tab + i;
-    const float4 cx = row[0];
-    const float4 cy = row[LENSED_EPL_ROWS];
-    float ox = cx.x + f * (cx.y + f * (cx.z + f * cx.w));
-    float oy = cy.x + f * (cy.y + f * (cy.z + f * cy.w));
-    return float2(copysign(ox, u.x), copysign(oy, u.y));
+// W at the diamond angle of u, with the signs of u (s = |u_x| + |u_y|)
+inline float2 lensed_epl_w_tab(float2 u, float s, const float4* tab) {
+    // p NT in [0, NT]; at u = 0 it is 0 * inf = NaN, which fmin turns into
+    // NT (the end row: any finite W, since the power factor s^(1-t) decides)
+    float x = fminf(fabs(u.y) * ((float)LENSED_EPL_NT / s), (float)LENSED_EPL_NT);
+    int i = (int)x;
+    float f = lensed_fract(x);
+    const float4 cx = tab[LENSED_EPL_AT(0, i)];
+    const float4 cy = tab[LENSED_EPL_AT(1, i)];
+    float wx = cx.x + f * (cx.y + f * (cx.z + f * cx.w));
+    float wy = cy.x + f * (cy.y + f * (cy.z + f * cy.w));
+    return float2(copysign(wx, u.x), copysign(wy, u.y));
 }
 
 // the pair of the offset r = x - c at position x: (F r, x) without shear,
@@ -78,12 +77,13 @@ inline float4 lensed_epl_pre_shear(const D* self, float2 r, float2 x) {
 }
 
 // the EPL deflection (without the shear) from the pair, in the lens frame
-// (v, with a = bwd v) and in the image frame
+// (v, with a = bwd v) and in the image frame: s^(2 hexp) W with
+// s = |u_x| + |u_y| (at u = 0: 0 for t < 1, else not finite, as the series)
 template <class D>
 inline float2 lensed_epl_table_vector_pre(const D* self, float4 pre, const float4* tab) {
     float2 u = float2(pre.x, pre.y);
-    float uu = dot(u, u);
-    return powr(uu, self->hexp) * lensed_epl_omega_tab(u, tab);
+    float s = fabs(u.x) + fabs(u.y);
+    return exp2((2.0f * self->hexp) * log2(s)) * lensed_epl_w_tab(u, s, tab);
 }
 
 template <class D>

@@ -166,11 +166,11 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
 }
 
 // -------------------------------------------------------------- EPL tables
-// One thread per (candidate, octant, row): the angular factor Omega and
-// dOmega/dtau of the tenth-order series at both ends of the interval, in
-// FP64 from the candidate's (q, t), turned into cubic Hermite coefficients
-// (epl_series.h); the last row of an octant is the constant value at tau = 1.  With `only` set, just the flagged candidates (the
-// emulator's fallback).
+// One thread per (candidate, row): the tabulated factor W and dW/dp (the
+// tenth-order series times the radial factor, epl_series.h) at both ends of
+// the interval, in FP64 from the candidate's (q, t), turned into cubic
+// Hermite coefficients; the last row is the constant value at p = 1.  With
+// `only` set, just the flagged candidates (the emulator's fallback).
 extern "C" __global__ __launch_bounds__(256) void lensed_epl_tables(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const float* __restrict__ params, float4* __restrict__ tables, const int* __restrict__ only)
@@ -180,24 +180,23 @@ extern "C" __global__ __launch_bounds__(256) void lensed_epl_tables(
     int s = lensed_find_sys<4>(sys, nsys, item);
     const lensed_desc& d = sys[s];
     int local = item - d.item_t;
-    int cand = local / (2 * LENSED_EPL_ROWS);
-    int rem = local - cand * 2 * LENSED_EPL_ROWS;
-    int half = rem / LENSED_EPL_ROWS, i = rem - half * LENSED_EPL_ROWS;
+    int cand = local / LENSED_EPL_ROWS;
+    int i = local - cand * LENSED_EPL_ROWS;
     if (only && !only[d.table_off / LENSED_EPL_TABLE_F4 + cand]) return;
     const float* p = params + d.param_off + (long long)cand * d.npar;
     double q = p[d.epl_q], t = p[d.epl_t];
     double x0, y0, dx0, dy0, x1, y1, dx1, dy1;
     float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
-    float4& ox = out[LENSED_EPL_AT(half, 0, i)];
-    float4& oy = out[LENSED_EPL_AT(half, 1, i)];
-    if (i == LENSED_EPL_NT) {  // the end point tau = 1 as a constant row
-        lensed_epl_node(q, t, half, 1.0, x1, y1, dx1, dy1);
+    float4& ox = out[LENSED_EPL_AT(0, i)];
+    float4& oy = out[LENSED_EPL_AT(1, i)];
+    if (i == LENSED_EPL_NT) {  // the end point p = 1 as a constant row
+        lensed_epl_node(q, t, 1.0, x1, y1, dx1, dy1);
         ox = float4((float)x1, 0.0f, 0.0f, 0.0f);
         oy = float4((float)y1, 0.0f, 0.0f, 0.0f);
         return;
     }
-    lensed_epl_node(q, t, half, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
-    lensed_epl_node(q, t, half, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
+    lensed_epl_node(q, t, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
+    lensed_epl_node(q, t, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
     ox = lensed_hermite(x0, dx0, x1, dx1);
     oy = lensed_hermite(y0, dy0, y1, dy1);
 }

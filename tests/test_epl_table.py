@@ -1,14 +1,15 @@
 """The tabulated EPL angular factor: the cubic-Hermite scheme of
-lensed_amd/kernels/epl_table.h (two octants in tau = min/max of |u|, 128
-intervals each, FP64 nodes) restated in numpy and checked against the
-oracle's tenth-order series, including the reflection symmetries that map
-every quadrant to the first."""
+lensed_amd/kernels/epl_table.h (the diamond angle p = |u_y| / (|u_x| +
+|u_y|), 256 intervals, FP64 nodes of W = g^h Omega with g = p^2 + (1-p)^2,
+h = (1-t)/2, so that |u|^(1-t) Omega = (|u_x| + |u_y|)^(1-t) W) restated in
+numpy and checked against the oracle's tenth-order series, including the
+reflection symmetries that map every quadrant to the first."""
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 
-NT = 128
+NT = 256
 
 
 def omega_and_derivative(q, t, psi):
@@ -26,42 +27,43 @@ def omega_and_derivative(q, t, psi):
 
 
 def table(q, t):
-    tab = np.zeros((2, NT, 2, 4))
-    for half in range(2):
-        tau = np.arange(NT + 1) / NT
-        a = np.arctan(tau)
-        psi = np.pi / 2 - a if half else a
-        ox, oy, gx, gy = omega_and_derivative(q, t, psi)
-        dpsi = (-1.0 if half else 1.0) / (1 + tau ** 2) / NT
-        for comp, (v, g) in enumerate(((ox, gx * dpsi), (oy, gy * dpsi))):
-            v0, v1, d0, d1 = v[:-1], v[1:], g[:-1], g[1:]
-            tab[half, :, comp] = np.stack([v0, d0, 3 * (v1 - v0) - 2 * d0 - d1, 2 * (v0 - v1) + d0 + d1], 1)
+    h = (1 - t) / 2
+    p = np.arange(NT + 1) / NT
+    g = p ** 2 + (1 - p) ** 2
+    ox, oy, gx, gy = omega_and_derivative(q, t, np.arctan2(p, 1 - p))
+    gh, lg = g ** h, h * (4 * p - 2) / g
+    tab = np.zeros((2, NT + 1, 4))
+    for comp, (o, go) in enumerate(((ox, gx), (oy, gy))):
+        v, d = gh * o, gh * (lg * o + go / g) / NT  # W and dW/df
+        v0, v1, d0, d1 = v[:-1], v[1:], d[:-1], d[1:]
+        tab[comp, :NT] = np.stack([v0, d0, 3 * (v1 - v0) - 2 * d0 - d1, 2 * (v0 - v1) + d0 + d1], 1)
+        tab[comp, NT, 0] = v[-1]
     return tab.astype(np.float32)
 
 
-def lookup(tab, u):
+def deflection_vector(tab, u, t):
+    """|u|^(1-t) Omega from the table: s^(1-t) W(p), s = |u_x| + |u_y|."""
     au, av = np.abs(u[:, 0]), np.abs(u[:, 1])
-    tau = np.minimum(au, av) / np.maximum(au, av)
-    s = tau * NT
-    i = np.minimum(s.astype(int), NT - 1)
-    f = s - i
-    half = (av > au).astype(int)
-    c = tab[half, i]
-    ox = c[:, 0, 0] + f * (c[:, 0, 1] + f * (c[:, 0, 2] + f * c[:, 0, 3]))
-    oy = c[:, 1, 0] + f * (c[:, 1, 1] + f * (c[:, 1, 2] + f * c[:, 1, 3]))
-    return np.stack([np.copysign(ox, u[:, 0]), np.copysign(oy, u[:, 1])], 1)
+    s = au + av
+    x = av / s * NT
+    i = x.astype(int)
+    f = x - i
+    c = tab[:, i]
+    w = c[:, :, 0] + f * (c[:, :, 1] + f * (c[:, :, 2] + f * c[:, :, 3]))
+    return s[:, None] ** (1 - t) * np.stack([np.copysign(w[0], u[:, 0]), np.copysign(w[1], u[:, 1])], 1)
 
 
 def test_table_interpolation_matches_series():
     rng = np.random.default_rng(0)
     worst = 0.0
-    for q, t in [(0.3, 0.5), (0.3, 1.5), (0.6, 1.0), (0.95, 1.2), (1.0, 0.8), (0.31, 1.49)]:
+    for q, t in [(0.3, 0.5), (0.3, 1.5), (0.6, 1.0), (0.95, 1.2), (1.0, 0.8), (0.31, 1.49), (0.25, 1.9)]:
         tab = table(q, t)
         phi = rng.uniform(-np.pi, np.pi, 4000)
-        u = np.stack([np.cos(phi), np.sin(phi)], 1) * rng.uniform(0.1, 3.0, (4000, 1))
-        got = lookup(tab, u)
-        exact = O.epl_omega(q, t, phi)
-        worst = max(worst, np.max(np.abs(got - exact)))
+        r = rng.uniform(0.1, 3.0, 4000)
+        u = np.stack([np.cos(phi), np.sin(phi)], 1) * r[:, None]
+        got = deflection_vector(tab, u, t)
+        exact = O.epl_omega(q, t, phi) * (r ** (1 - t))[:, None]
+        worst = max(worst, np.max(np.abs(got - exact) / (r ** (1 - t))[:, None]))
     # float32 coefficients: a few ulp of |Omega| <= ~3
     assert worst < 2e-6, worst
 
@@ -75,6 +77,11 @@ def test_series_derivative_is_consistent():
     b = O.epl_omega(q, t, psi - h)
     assert np.allclose((a - b)[:, 0] / (2 * h), dx, atol=1e-7)
     assert np.allclose((a - b)[:, 1] / (2 * h), dy, atol=1e-7)
+    # and the tabulated W(p) = g^h Omega(psi(p)): its f-derivative a1 against
+    # a central difference of the table's own values a0 one interval apart
+    tab = table(q, t).astype(np.float64)
+    fd = (tab[:, 2:, 0] - tab[:, :-2, 0]) / 2
+    assert np.allclose(tab[:, 1:-1, 1], fd, atol=3e-5)
 
 
 # ------------------------------------------------------------------------
@@ -103,29 +110,31 @@ inline float4 make_float4(float a, float b, float c, float d) { return float4(a,
 @SERIES@
 }
 static void build_table(double q, double t, lcl::float4* tab) {
-    for (int half = 0; half < 2; ++half)
-        for (int i = 0; i <= LENSED_EPL_NT; ++i) {
-            double x0, y0, dx0, dy0, x1, y1, dx1, dy1;
-            lcl::float4& ox = tab[LENSED_EPL_AT(half, 0, i)];
-            lcl::float4& oy = tab[LENSED_EPL_AT(half, 1, i)];
-            if (i == LENSED_EPL_NT) {
-                hostseries::lensed_epl_node(q, t, half, 1.0, x1, y1, dx1, dy1);
-                ox = lcl::float4((float)x1, 0.f, 0.f, 0.f);
-                oy = lcl::float4((float)y1, 0.f, 0.f, 0.f);
-                continue;
-            }
-            hostseries::lensed_epl_node(q, t, half, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
-            hostseries::lensed_epl_node(q, t, half, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
-            ox = hostseries::lensed_hermite(x0, dx0, x1, dx1);
-            oy = hostseries::lensed_hermite(y0, dy0, y1, dy1);
+    for (int i = 0; i <= LENSED_EPL_NT; ++i) {
+        double x0, y0, dx0, dy0, x1, y1, dx1, dy1;
+        lcl::float4& ox = tab[LENSED_EPL_AT(0, i)];
+        lcl::float4& oy = tab[LENSED_EPL_AT(1, i)];
+        if (i == LENSED_EPL_NT) {
+            hostseries::lensed_epl_node(q, t, 1.0, x1, y1, dx1, dy1);
+            ox = lcl::float4((float)x1, 0.f, 0.f, 0.f);
+            oy = lcl::float4((float)y1, 0.f, 0.f, 0.f);
+            continue;
         }
+        hostseries::lensed_epl_node(q, t, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
+        hostseries::lensed_epl_node(q, t, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
+        ox = hostseries::lensed_hermite(x0, dx0, x1, dx1);
+        oy = hostseries::lensed_hermite(y0, dy0, y1, dy1);
+    }
 }
-extern "C" void host_omega(double q, double t, const float* u, int n, float* out) {
+// the lens-frame deflection vector |u|^(1-t) Omega of lensed_epl_table_vector_pre
+struct HostEpl { float hexp; };
+extern "C" void host_vector(double q, double t, const float* u, int n, float* out) {
     static lcl::float4 tab[LENSED_EPL_TABLE_F4];
     build_table(q, t, tab);
+    const HostEpl e{(float)(0.5 * (1 - t))};
     for (int k = 0; k < n; ++k) {
-        lcl::float2 o = lcl::lensed_epl_omega_tab(lcl::float2(u[2 * k], u[2 * k + 1]), tab);
-        out[2 * k] = o.x; out[2 * k + 1] = o.y;
+        lcl::float2 v = lcl::lensed_epl_table_vector_pre(&e, lcl::float4(u[2 * k], u[2 * k + 1], 0.f, 0.f), tab);
+        out[2 * k] = v.x; out[2 * k + 1] = v.y;
     }
 }
 // source-plane position of the ray at pixel p plus node q behind object 0
@@ -161,7 +170,7 @@ def _host_lib():
     subprocess.check_call(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-w", "-o", so, cpp])
     h = ctypes.CDLL(so)
     fp = POINTER(c_float)
-    h.host_omega.argtypes = [c_double, c_double, fp, c_int, fp]
+    h.host_vector.argtypes = [c_double, c_double, fp, c_int, fp]
     h.host_trace.argtypes = [fp, fp, fp, c_int, fp]
     return h
 
@@ -174,18 +183,23 @@ def test_product_table_lookup_on_the_host():
         phi = rng.uniform(-np.pi, np.pi, 3000)
         r = rng.uniform(0.05, 5.0, 3000)
         u = np.stack([r * np.cos(phi), r * np.sin(phi)], 1)
-        # exact diagonals (tau = 1, the end row) and axes (tau = 0)
+        # diagonals (p = 1/2), axes (p = 0 and the end row p = 1)
         u[:8] = [[1, 1], [-2, 2], [3, -3], [-0.5, -0.5], [1, 0], [0, -2], [-3, 0], [0, 4]]
         phi[:8] = np.arctan2(u[:8, 1], u[:8, 0])
+        r[:8] = np.hypot(u[:8, 0], u[:8, 1])
         u = u.astype(np.float32)
         out = np.zeros_like(u)
-        h.host_omega(q, t, u.ctypes.data_as(fp), len(u), out.ctypes.data_as(fp))
-        exact = O.epl_omega(q, t, phi)
-        assert np.max(np.abs(out - exact)) < 2e-6, (q, t, np.max(np.abs(out - exact)))
-    # u = 0 gives the tau = 0 row, not a NaN
+        h.host_vector(q, t, u.ctypes.data_as(fp), len(u), out.ctypes.data_as(fp))
+        exact = O.epl_omega(q, t, phi) * (r ** (1 - t))[:, None]
+        err = np.max(np.abs(out - exact) / (r ** (1 - t))[:, None])
+        assert err < 3e-6, (q, t, err)
+    # u = 0: the deflection of the series, 0 for t < 1 and not finite beyond
     z = np.zeros((1, 2), np.float32)
-    h.host_omega(0.6, 1.2, z.ctypes.data_as(fp), 1, z.ctypes.data_as(fp))
-    assert np.all(np.isfinite(z))
+    h.host_vector(0.6, 0.8, z.ctypes.data_as(fp), 1, z.ctypes.data_as(fp))
+    assert np.all(z == 0)
+    z = np.zeros((1, 2), np.float32)
+    h.host_vector(0.6, 1.2, z.ctypes.data_as(fp), 1, z.ctypes.data_as(fp))
+    assert not np.all(np.isfinite(z))
 
 
 def test_split_pair_ray_tracing_matches_oracle():
