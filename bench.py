@@ -46,9 +46,9 @@ PEAK_HBM = 8000.0  # GB/s
 RAY_VALU = 35
 VALU_ISSUE_PER_S = 256 * 2.4e9  # wave instructions per second
 # HBM bytes per 4096-candidate render launch, measured with rocprofv3 --pmc:
-# FETCH_SIZE 140300 KiB (x 2, the factor a 1 GiB copy calibrates) in and
+# FETCH_SIZE 165455 KiB (x 2, the factor a 1 GiB copy calibrates) in and
 # WRITE_SIZE 1048576 KiB (x 1) out, i.e. the 4096 model images
-TRAFFIC_RENDER_4096 = (140300 * 2 + 1048576) * 1024
+TRAFFIC_RENDER_4096 = (165455 * 2 + 1048576) * 1024
 
 
 def make_workload(K):
