@@ -136,7 +136,8 @@ enum {
     LENSED_PROFILE = 1, /* per-stage hipEvent timing, the --profile stages */
     LENSED_EMULATE = 2, /* EPL angular factor from the trained emulator table */
     LENSED_TABLE = 4,   /* EPL angular factor from the exact-series table */
-    LENSED_GRAPH = 8    /* replay batched launches from a captured hipGraph */
+    LENSED_GRAPH = 8,   /* replay batched launches from a captured hipGraph */
+    LENSED_NO_CULL = 16 /* trace every ray (no skipping of pixels whose rays provably all miss the source) */
 };
 
 /* One observed system: data as prepared by src/lensed.c:406-511. */

@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(HERE, "liblensed_amd.so")
 OBJ_LENS, OBJ_SOURCE, OBJ_FOREGROUND = ord("L"), ord("S"), ord("F")
 (PAR_PARAMETER, PAR_POSITION_X, PAR_POSITION_Y, PAR_RADIUS, PAR_MAGNITUDE, PAR_AXIS_RATIO,
  PAR_POS_ANGLE) = range(7)
-LENSED_PROFILE, LENSED_EMULATE, LENSED_TABLE, LENSED_GRAPH = 1, 2, 4, 8
+LENSED_PROFILE, LENSED_EMULATE, LENSED_TABLE, LENSED_GRAPH, LENSED_NO_CULL = 1, 2, 4, 8, 16
 
 
 class Param(Structure):

@@ -35,6 +35,7 @@ struct System {
     std::string rule;
     std::vector<float> quad;    // float4 nodes (dx, dy, w, e), scaled
     int nq = 0;
+    float qrad = 0;             // largest node distance from the pixel centre
     std::vector<ModelObject> objs;
     std::vector<Param> pars;    // object order
     std::vector<size_t> pmap;   // sampler order -> object order
@@ -51,6 +52,7 @@ struct System {
     // EPL angular table: index of the EPL object and of its q, t parameters
     int epl_obj = -1, epl_q = -1, epl_t = -1;
     bool table = false;
+    bool cull = false;          // pixels culled before the render (model_culls)
     std::vector<float> h_image, h_weight;  // host copies for the output layers
     // dumper results (object order)
     std::vector<double> mean, sigma, ml, map;
@@ -65,6 +67,7 @@ struct Layout {
     int n_nodes = 0, n_tcands = 0;  // emulator: table nodes and candidates with a table
     size_t lds_conv = 0, lds_render = 0;
     bool conv_small = false;        // short tiles: launch too small to fill the GPU
+    bool any_cull = false;          // some system culls pixels before the render (lensed_cull)
     int grid_conv = 0, grid_render = 0;  // resident blocks for this launch's LDS use
     lensed_desc* d_desc = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -79,7 +82,7 @@ struct lensed {
     std::vector<lsd::System> sys;
     hipModule_t module = nullptr;
     hipFunction_t f_set = nullptr, f_render = nullptr, f_conv = nullptr, f_conv_small = nullptr, f_final = nullptr,
-                  f_tables = nullptr;
+                  f_tables = nullptr, f_cull = nullptr;
     int cu_count = 1;
     hipStream_t stream = nullptr;
     int grid_render = 0, grid_conv = 0, grid_conv_small = 0;
@@ -114,6 +117,8 @@ struct lensed {
     std::string output;          // FITS file the dumper writes (empty: none)
     void* d_npre = nullptr;      // per candidate and node: EPL lens-frame pair of the node offset
     size_t cap_npre = 0;
+    int* d_trace = nullptr;      // per render item and wave: trace its rays (lensed_cull)
+    size_t cap_trace = 0;
     // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem
     hipEvent_t ev[9] = {};
     double prof_ms[8] = {};

@@ -124,8 +124,9 @@ std::string translate_object(const std::string& src, const std::string& mangled)
     return out;
 }
 
-static const char* const kObjectMacros[] = {"type",       "params",       "data",         "deflection",   "brightness",
-                                             "foreground", "set",          "frame_centre", "frame_map",    "frame_profile"};
+static const char* const kObjectMacros[] = {"type",         "params",    "data",          "deflection",
+                                             "brightness",   "foreground", "set",          "frame_centre",
+                                             "frame_map",    "frame_profile", "frame_cutoff"};
 
 static std::string undef_macros() {
     std::string s;
@@ -137,7 +138,7 @@ static std::string undef_macros() {
 // with frame_map linear: brightness(x) = frame_profile(u)
 static std::string frame_macros(const std::string& m) {
     return "#define frame_centre frame_centre_" + m + "\n#define frame_map frame_map_" + m +
-           "\n#define frame_profile frame_profile_" + m + "\n";
+           "\n#define frame_profile frame_profile_" + m + "\n#define frame_cutoff frame_cutoff_" + m + "\n";
 }
 
 std::string wrap_object(const std::string& translated, const std::string& m, const std::string& name) {
@@ -243,18 +244,29 @@ static std::string data_ptr(const std::string& m, size_t off, const char* base) 
     return o.str();
 }
 
-static bool source_has_frame(const std::string& name) {
+// whether a plugin defines the functions named by the patterns
+static bool plugin_defines(const std::string& name, const std::vector<const char*>& patterns) {
     static std::mutex mu;
-    static std::map<std::string, bool> known;
+    static std::map<std::string, std::string> sources;
     std::lock_guard<std::mutex> lock(mu);
-    auto it = known.find(name);
-    if (it != known.end()) return it->second;
-    std::string src;
-    bool ok = load_object_source(name, &src, nullptr);
-    for (const char* re : {R"(static\s+float2\s+frame_centre\s*\()", R"(static\s+float2\s+frame_map\s*\()",
-                           R"(static\s+float\s+frame_profile\s*\()"})
-        ok = ok && std::regex_search(src, std::regex(re));
-    return known[name] = ok;
+    auto it = sources.find(name);
+    if (it == sources.end()) {
+        std::string src;
+        if (!load_object_source(name, &src, nullptr)) src.clear();
+        it = sources.emplace(name, src).first;
+    }
+    bool ok = !it->second.empty();
+    for (const char* re : patterns) ok = ok && std::regex_search(it->second, std::regex(re));
+    return ok;
+}
+
+static bool source_has_frame(const std::string& name) {
+    return plugin_defines(name, {R"(static\s+float2\s+frame_centre\s*\()", R"(static\s+float2\s+frame_map\s*\()",
+                                 R"(static\s+float\s+frame_profile\s*\()"});
+}
+
+static bool source_has_cutoff(const std::string& name) {
+    return plugin_defines(name, {R"(static\s+float\s+frame_cutoff\s*\()"});
 }
 
 // The lens-frame form of a tabulated EPL lens applies when that lens is the
@@ -273,6 +285,11 @@ static int frame_source(const std::vector<ModelObject>& objs) {
     }
     if (nlens != 1 || table < 0 || nbehind != 1 || !source_has_frame(objs[src].name)) return -1;
     return src;
+}
+
+bool model_culls(const std::vector<ModelObject>& objs) {
+    const int fsrc = frame_source(objs);
+    return fsrc >= 0 && objs.size() == 2 && source_has_cutoff(objs[fsrc].name);
 }
 
 std::string compute_function(const std::vector<ModelObject>& objs, int k) {
@@ -366,6 +383,28 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
     }
     if (plane_open) close_plane();
     o << "    return f;\n}\n";
+    // pixel culling (lensed_frame_cull): only when the model is the lens and
+    // the framed source alone, so a skipped pixel's value is exactly 0
+    o << "static inline bool cull_" << k << "(uint* data, float4 base, const float4* tab, float rho)\n{\n"
+      << "    (void)data; (void)base; (void)tab; (void)rho;\n";
+    if (model_culls(objs)) {
+        const bool shear = pre_fn == "lensed_epl_pre_shear";
+        o << "    const struct data_" << lm << "* s = " << data_ptr(lm, lens_off, "data") << ";\n"
+          << "    struct data_" << sm << "* g = " << data_ptr(sm, src_off, "data") << ";\n"
+          << "    const float2 n0 = frame_map_" << sm << "(g, float2(-s->bwd.x, -s->bwd.z));\n"
+          << "    const float2 n1 = frame_map_" << sm << "(g, float2(-s->bwd.y, -s->bwd.w));\n"
+          << "    const float2 t0 = frame_map_" << sm << "(g, float2(1.0f, 0.0f));\n"
+          << "    const float2 t1 = frame_map_" << sm << "(g, float2(0.0f, 1.0f));\n"
+          << "    const float lnorm = "
+          << (shear ? "lensed_frob(float4(1.0f - s->shear.x, -s->shear.y, -s->shear.z, 1.0f - s->shear.w))"
+                    : "1.41421356f")
+          << ";\n"
+          << "    return lensed_frame_cull(s, base, tab, n0, n1, sqrtf(dot(t0, t0) + dot(t1, t1)), lnorm, frame_cutoff_"
+          << sm << "(g), rho);\n";
+    } else {
+        o << "    return false;\n";
+    }
+    o << "}\n";
     return o.str();
 }
 

@@ -53,6 +53,9 @@ struct ModelObject {
 // Generated code for one model (ordered object list): compute_<k>() and
 // set_params_<k>().  Offsets are in 4-byte words.
 std::string compute_function(const std::vector<ModelObject>& objs, int k);
+// whether the model's pixels can be culled before the render: a tabulated
+// EPL lens and one source with a frame and a cutoff, nothing else
+bool model_culls(const std::vector<ModelObject>& objs);
 std::string set_params_function(const std::vector<ModelObject>& objs, int k);
 
 // Total data words of a model.

@@ -38,9 +38,11 @@ struct lensed_desc {
     int quad_off, nq, conv_ry, sep_rank;  // nodes (float4 offset, count); conv rows/thread; PSF rank (0 = direct)
     long long sep_off;                   // floats into psf[]: per rank, column [psfh] then row [psfw] factor
     int npart, table_mode, item_e, pad4;  // chi^2 partials per candidate (8-row x 64-column blocks); EPL table
-                                          // in use; first emulator node item (2 (NT + 1) nodes per candidate)
+                                          // in use; first emulator node item (NT + 1 nodes per candidate)
     long long psft_off;                  // floats into psf[]: the taps transposed, [psfw][psfh]
     long long npre_off;                  // float4 into npre[]: the nodes' EPL lens-frame pairs ([ncand][nq])
+    float qrad;                          // largest distance of a quadrature node from the pixel centre
+    int cull;                            // skip pixels whose rays provably all miss the source
 };
 
 // LDS row stride of the conv tile: 64 + PW - 1 columns plus room for the

@@ -115,6 +115,49 @@ inline float2 lensed_frame_trace(float4 pre, float2 n0, float2 n1, float2 v) {
     return float2(fma(v.y, n1.x, fma(v.x, n0.x, pre.z)), fma(v.y, n1.y, fma(v.x, n0.y, pre.w)));
 }
 
+// ------------------------------------------------------------------ culling
+// A pixel whose rays all land where the source's brightness is exactly 0
+// need not be traced: its value is 0 either way, bit for bit.  For the
+// pixel centre x_c (the pixel part of the pair, `base`) and any of its rays
+// x = x_c + q, |q| <= rho, with y = x - S (x - c) - bwd v(u), u = F (x - c),
+// v = |u|^(1-t) Omega(psi):
+//     |y(x) - y(x_c)| <= |I - S| rho + |bwd| sup|dv/du| |F| rho,
+//     |dv/du| <= |u|^(-t) (|1-t| A + B),  A = sum |C_k|, B = sum (2k+1) |C_k|
+// (the radial and the angular derivative of the series), the sup taken over
+// |u| within |F| rho of |u_c| (a pixel whose rays can reach u = 0 is never
+// skipped).  Norms are Frobenius norms, which bound the operator norms.  In
+// the source's frame u_s = T (y - c_s) every ray is then within |T| times
+// that of the centre ray's u_s, and the pixel is skipped when even the
+// nearest is beyond the source's cutoff (frame_cutoff) by a margin of 1:
+// the FP32 rounding of the traced positions and the table's interpolation
+// error stay far below that.
+template <class D>
+inline float lensed_epl_slope_bound(const D* self) {
+    float a = 1.0f, b = 1.0f;
+    for (int k = 0; k < 10; ++k) {
+        const float ck = fabsf(self->coef[k]);
+        a += ck;
+        b += (float)(2 * k + 3) * ck;
+    }
+    return fabsf(2.0f * self->hexp) * a + b;
+}
+
+inline float lensed_frob(mat22 m) { return sqrtf(m.x * m.x + m.y * m.y + m.z * m.z + m.w * m.w); }
+
+template <class D>
+inline bool lensed_frame_cull(const D* self, float4 base, const float4* tab, float2 n0, float2 n1, float tnorm,
+                              float lnorm, float cutoff, float rho) {
+    const float2 us =
+        lensed_frame_trace(base, n0, n1, lensed_clamp_deflection(lensed_epl_table_vector_pre(self, base, tab)));
+    const float rc = sqrtf(base.x * base.x + base.y * base.y);
+    const float fr = lensed_frob(self->fwd) * rho;
+    const float rlo = rc - fr, rhi = rc + fr;
+    const float t = 1.0f - 2.0f * self->hexp;
+    const float pw = fmaxf(exp2(-t * log2(rlo)), exp2(-t * log2(rhi)));  // sup of |u|^(-t)
+    const float dy = lnorm * rho + lensed_frob(self->bwd) * lensed_epl_slope_bound(self) * pw * fr;
+    return rlo > 0.0f && sqrtf(us.x * us.x + us.y * us.y) - tnorm * dy > cutoff + 1.0f;
+}
+
 }  // namespace lcl
 #ifdef LCL_GPU
 #pragma clang force_cuda_host_device end

@@ -87,10 +87,11 @@ extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
 // node table in scalar registers and the candidate's object block held in
 // SGPRs (wave-uniform), so the per-ray work is pure VALU.
 template <int K, bool ERR>
-__device__ inline void lensed_render_item(const lensed_desc& d, int cand, int tile,
+__device__ inline void lensed_render_item(const lensed_desc& d, int item, int cand, int tile,
                                           const u32* __restrict__ blocks, const float4* __restrict__ quad,
                                           const float4* __restrict__ tables, const float4* __restrict__ npre,
-                                          float4* tab_lds, float* __restrict__ model, float* __restrict__ errp)
+                                          const int* __restrict__ trace, float4* tab_lds,
+                                          float* __restrict__ model, float* __restrict__ errp)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
@@ -115,14 +116,20 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
             // pixel's part and the node's (from set_params, scalar loads)
             const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
             const float4* np_c = npre + d.npre_off + (long long)cand * d.nq;
-            for (int n = 0; n < d.nq; ++n) {
-                const float4 node = q[n];
-                const float4 np = np_c[n];
-                const lcl::float4 pre(base.x + np.x, base.y + np.y, base.z + np.z, base.w + np.w);
-                float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y),
-                                                (const lcl::float4*)tab_lds, pre);
-                f += node.z * v;
-                if (ERR) e += node.w * v;
+            // waves whose pixels all provably miss the source (lensed_cull) skip
+            // their rays: the pixels' values are 0 either way
+            const int wave = (int)threadIdx.x / 64;
+            const bool go = !d.cull || __builtin_amdgcn_readfirstlane(trace[(long long)item * (LENSED_RENDER_BS / 64) + wave]);
+            if (go) {
+                for (int n = 0; n < d.nq; ++n) {
+                    const float4 node = q[n];
+                    const float4 np = np_c[n];
+                    const lcl::float4 pre(base.x + np.x, base.y + np.y, base.z + np.z, base.w + np.w);
+                    float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y),
+                                                    (const lcl::float4*)tab_lds, pre);
+                    f += node.z * v;
+                    if (ERR) e += node.w * v;
+                }
             }
         } else {
             for (int n = 0; n < d.nq; ++n) {
@@ -140,10 +147,13 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
     if (d.table_mode) __syncthreads();  // the table is replaced by the next item
 }
 
-extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
+// eight waves per SIMD (at most 64 VGPRs): the ray loop's latencies
+// (transcendentals, LDS lookups) are hidden by the other waves
+extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void lensed_render(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const u32* __restrict__ blocks, const float4* __restrict__ quad, const float4* __restrict__ tables,
-    const float4* __restrict__ npre, float* __restrict__ model, float* __restrict__ errp)
+    const float4* __restrict__ npre, const int* __restrict__ trace, float* __restrict__ model,
+    float* __restrict__ errp)
 {
     extern __shared__ float4 tab_lds[];
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
@@ -155,11 +165,72 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_render(
         switch (d.model) {
 #define LENSED_RENDER_CASE(K)                                                                            \
     case K:                                                                                              \
-        if (errp) lensed_render_item<K, true>(d, cand, tile, blocks, quad, tables, npre, tab_lds, model, errp); \
-        else lensed_render_item<K, false>(d, cand, tile, blocks, quad, tables, npre, tab_lds, model, errp);     \
+        if (errp)                                                                                        \
+            lensed_render_item<K, true>(d, item, cand, tile, blocks, quad, tables, npre, trace, tab_lds, model, \
+                                        errp);                                                           \
+        else                                                                                             \
+            lensed_render_item<K, false>(d, item, cand, tile, blocks, quad, tables, npre, trace, tab_lds, model, \
+                                         errp);                                                          \
         break;
             LENSED_FOR_MODELS(LENSED_RENDER_CASE)
 #undef LENSED_RENDER_CASE
+            default: break;
+        }
+    }
+}
+
+// -------------------------------------------------------------------- cull
+// Before the render: for each render item and wave, whether any of its
+// pixels can receive light (lensed_frame_cull in epl_table.h bounds where a
+// pixel's rays land).  Same items, grid and staged table as the render, so
+// the render only reads one flag per wave and keeps its registers for the
+// rays.  Only for systems with d.cull (a tabulated EPL lens and one framed
+// source with a cutoff).
+template <int K>
+__device__ inline void lensed_cull_item(const lensed_desc& d, int item, int cand, int tile,
+                                        const u32* __restrict__ blocks, const float4* __restrict__ tables,
+                                        float4* tab_lds, int* __restrict__ trace)
+{
+    u32 blk[LENSED_DWORDS];
+    const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
+#pragma unroll
+    for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = i < d.dwords ? __builtin_amdgcn_readfirstlane(gb[i]) : 0u;
+    const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
+    for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
+    __syncthreads();
+    const int k = tile * LENSED_RENDER_BS + (int)threadIdx.x;
+    bool lit = false;
+    if (k < d.npix) {
+        const int py = k / d.width, px = k - py * d.width;
+        const float cx = d.pcs[0] + d.pcs[2] * (float)px;
+        const float cy = d.pcs[1] + d.pcs[3] * (float)py;
+        const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
+        lit = !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, d.qrad);
+    }
+    const bool any = __any(lit);
+    if ((threadIdx.x & 63) == 0) trace[(long long)item * (LENSED_RENDER_BS / 64) + threadIdx.x / 64] = any ? 1 : 0;
+    __syncthreads();  // the table is replaced by the next item
+}
+
+extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems,
+    const u32* __restrict__ blocks, const float4* __restrict__ tables, int* __restrict__ trace)
+{
+    extern __shared__ float4 tab_lds[];
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        int s = lensed_find_sys<0>(sys, nsys, item);
+        const lensed_desc& d = sys[s];
+        if (!d.cull) continue;
+        int local = item - d.item_r;
+        int cand = local / d.ntile_r;
+        int tile = local - cand * d.ntile_r;
+        switch (d.model) {
+#define LENSED_CULL_CASE(K)                                                             \
+    case K:                                                                             \
+        lensed_cull_item<K>(d, item, cand, tile, blocks, tables, tab_lds, trace);       \
+        break;
+            LENSED_FOR_MODELS(LENSED_CULL_CASE)
+#undef LENSED_CULL_CASE
             default: break;
         }
     }

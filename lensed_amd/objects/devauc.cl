@@ -41,6 +41,13 @@ static float frame_profile(local data* this, float2 u)
     return this->peak*exp(-DEVAUC_BN*sqrt(sqrt(s)));
 }
 
+// |u| beyond which frame_profile is exactly 0 in FP32 (below 2^-160)
+static float frame_cutoff(local data* this)
+{
+    float a = fmax(log(this->peak) + 111, 0.0f)/DEVAUC_BN;
+    return this->rs*(a*a)*(a*a);
+}
+
 static float brightness(local data* this, float2 x)
 {
     return frame_profile(this, frame_map(this, x - frame_centre(this)));

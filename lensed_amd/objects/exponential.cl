@@ -35,6 +35,12 @@ static float frame_profile(local data* this, float2 u)
     return this->peak*exp(-length(u)/this->rs);
 }
 
+// |u| beyond which frame_profile is exactly 0 in FP32 (below 2^-160)
+static float frame_cutoff(local data* this)
+{
+    return this->rs*fmax(log(this->peak) + 111, 0.0f);
+}
+
 static float brightness(local data* this, float2 x)
 {
     return frame_profile(this, frame_map(this, x - frame_centre(this)));

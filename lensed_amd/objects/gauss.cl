@@ -35,6 +35,12 @@ static float frame_profile(local data* this, float2 u)
     return exp2(fma(-u.x, u.x, fma(-u.y, u.y, this->lpk)));
 }
 
+// |u| beyond which frame_profile is exactly 0 in FP32 (its exponent below -160)
+static float frame_cutoff(local data* this)
+{
+    return sqrt(fmax(this->lpk + 160, 0.0f));
+}
+
 static float brightness(local data* this, float2 x)
 {
     return frame_profile(this, frame_map(this, x - frame_centre(this)));

@@ -41,6 +41,13 @@ static float frame_profile(local data* this, float2 u)
     return exp2(this->lnorm - 1.44269504088896340736f*exp2(this->lscale + this->hinv*log2(dot(u, u))));
 }
 
+// |u| beyond which frame_profile is exactly 0 in FP32 (its exponent below -160)
+static float frame_cutoff(local data* this)
+{
+    float e = (this->lnorm + 160)/1.44269504088896340736f;
+    return e > 0 ? exp2(0.5f*(log2(e) - this->lscale)/this->hinv) : 0.0f;
+}
+
 static float brightness(local data* this, float2 x)
 {
     return frame_profile(this, frame_map(this, x - frame_centre(this)));

@@ -333,3 +333,38 @@ def test_c3_epl_table_matches_exact(c3):
     model = oracle_model(wl.objects, truth, wl.shape, wl.psf)
     rel, _ = rel_err(planes["image"], model)
     assert rel <= 1e-4, rel
+
+
+def test_c3_culling_is_exact(c3):
+    """Pixels whose rays provably all land beyond the source's cutoff are not
+    traced (lensed_frame_cull): logL and model planes are bit for bit those
+    of tracing every ray -- over prior draws, near the truth, and with the
+    lens centre on a pixel centre (a ray through u = 0, t on both sides of
+    1 and equal to it); and the culled launch is the faster one."""
+    from lensed_amd._lib import LENSED_NO_CULL, LENSED_PROFILE, LENSED_TABLE
+    wl, pids = c3
+    phys = near_truth_phys(wl, pids, 64, 2e-4)
+    cubes = wl.candidates(14, 1024)
+    edge = np.array([wl.truth[k] for k in pids])[None].repeat(4, 0)
+    edge[:, pids.index("lens.x")] = 128.0
+    edge[:, pids.index("lens.y")] = 128.0
+    edge[:, pids.index("lens.t")] = [0.6, 0.999, 1.0, 1.4]
+    res, render_ms = [], []
+    for extra in (0, LENSED_NO_CULL):
+        with Context(wl.spec(), max_batch=1024, flags=LENSED_TABLE | LENSED_GRAPH | extra) as ctx:
+            lc, _ = ctx.loglike_batch(cubes)
+            ln = ctx.loglike_phys(np.concatenate([phys, edge]))
+            planes = [ctx.render(p) for p in (edge[0], edge[2], edge[3], phys[0])]
+            res.append((lc, ln, planes))
+        with Context(wl.spec(), max_batch=1024, flags=LENSED_TABLE | LENSED_PROFILE | extra) as ctx:
+            ctx.loglike_batch(cubes)
+            calls, ms = ctx.profile()
+            render_ms.append(ms["render"] / calls)
+    (lc, ln, pl), (lc0, ln0, pl0) = res
+    assert np.all(np.isfinite(lc)) and np.all(np.isfinite(ln))
+    assert np.array_equal(lc, lc0) and np.array_equal(ln, ln0)
+    for a, b in zip(pl, pl0):
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+    print(f"render ms per launch: culled {render_ms[0]:.3f}, every ray {render_ms[1]:.3f}", flush=True)
+    assert render_ms[0] < render_ms[1], render_ms
