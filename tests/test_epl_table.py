@@ -227,6 +227,28 @@ def test_split_pair_ray_tracing_matches_oracle():
 # u = T (x - S (x - c) - c_s) - (T bwd) v; whole rays against the oracle
 
 TABLE_HARNESS = r'''
+// pixels the cull skips, and how many of their rays (nodes q[n]) give a
+// brightness that is not exactly 0 (must be none)
+extern "C" void host_cull(const float* params, int w, int h, const float* q, int nq, float rho, int* out) {
+    static lcl::uint data[4096];
+    lcl::set_params_0(data, params);
+    static lcl::float4 tab[LENSED_EPL_TABLE_F4];
+    build_table(params[4], params[3], tab);
+    int skipped = 0, lit = 0;
+    for (int k = 0; k < w * h; ++k) {
+        const float cx = 1.0f + (float)(k % w), cy = 1.0f + (float)(k / w);
+        const lcl::float4 b = lcl::epl_pre_0(data, lcl::float2(cx, cy), true);
+        if (!lcl::cull_0(data, b, tab, rho)) continue;
+        ++skipped;
+        for (int n = 0; n < nq; ++n) {
+            const lcl::float4 d = lcl::epl_pre_0(data, lcl::float2(q[2 * n], q[2 * n + 1]), false);
+            const lcl::float4 pre(b.x + d.x, b.y + d.y, b.z + d.z, b.w + d.w);
+            if (lcl::compute_0(data, lcl::float2(cx + q[2 * n], cy + q[2 * n + 1]), tab, pre) != 0.0f) ++lit;
+        }
+    }
+    out[0] = skipped;
+    out[1] = lit;
+}
 extern "C" void host_rays(const float* params, int w, int h, float dx, float dy, float* out) {
     static lcl::uint data[4096];
     lcl::set_params_0(data, params);
@@ -260,6 +282,7 @@ def _host_table_lib(source):
     h = ctypes.CDLL(so)
     fp = POINTER(c_float)
     h.host_rays.argtypes = [fp, c_int, c_int, c_float, c_float, fp]
+    h.host_cull.argtypes = [fp, c_int, c_int, fp, c_int, c_float, POINTER(c_int)]
     return h
 
 
@@ -299,3 +322,37 @@ def test_lens_frame_ray_through_the_lens_centre(t):
     assert np.all(np.isfinite(out))
     assert out[49, 49] == pytest.approx(ref[49, 49], abs=1e-6 * np.max(ref))  # pixel (50, 50)
     assert np.max(np.abs(out - ref)) < 2e-5 * np.max(ref)
+
+
+@pytest.mark.parametrize("source", ["gauss", "sersic", "exponential"])
+def test_cull_bound_never_skips_a_lit_ray(source):
+    """The pixel cull (lensed_frame_cull) is a proof that every ray of a
+    skipped pixel has brightness exactly 0; brute force on the host over many
+    random lenses and sources (including pixels next to the lens centre and
+    slopes on both sides of t = 1): every ray of every skipped pixel traced,
+    none may be lit; and some pixels must be skipped."""
+    from lensed_amd import quadrature as Q
+    h = _host_table_lib(source)
+    qq, _ = Q.rule("g3k7")
+    qn = np.ascontiguousarray(np.asarray(qq, np.float32).reshape(-1, 2))
+    rho = float(np.max(np.hypot(qn[:, 0], qn[:, 1]))) * (1 + 1e-6)
+    rng = np.random.default_rng(11)
+    fp = POINTER(c_float)
+    out = (c_int * 2)()
+    skipped = 0
+    for it in range(12):
+        lens = [rng.uniform(45, 56), rng.uniform(45, 56), rng.uniform(5, 15), rng.uniform(0.3, 1.7),
+                rng.uniform(0.3, 1.0), rng.uniform(0, 180), rng.uniform(-0.1, 0.1), rng.uniform(-0.1, 0.1)]
+        if it % 4 == 0:
+            lens[0], lens[1] = 50.0, 50.0  # the lens centre on a pixel centre
+        sx, sy = rng.uniform(40, 60), rng.uniform(40, 60)
+        src = {"gauss": [sx, sy, rng.uniform(0.5, 2.5), rng.uniform(-5, 5), rng.uniform(0.4, 1), rng.uniform(0, 180)],
+               "sersic": [sx, sy, rng.uniform(0.5, 3), rng.uniform(-5, 5), rng.uniform(0.5, 4), rng.uniform(0.4, 1),
+                          rng.uniform(0, 180)],
+               "exponential": [sx, sy, rng.uniform(0.3, 1.5), rng.uniform(-5, 5), rng.uniform(0.4, 1),
+                               rng.uniform(0, 180)]}[source]
+        params = np.array(lens + src, np.float32)
+        h.host_cull(params.ctypes.data_as(fp), 100, 100, qn.ctypes.data_as(fp), len(qn), rho, out)
+        assert out[1] == 0, (it, params, out[0], out[1])
+        skipped += out[0]
+    assert skipped > 100, skipped  # not a vacuous check (the exponential's zero radius is ~111 scale lengths)
