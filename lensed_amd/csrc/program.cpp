@@ -384,10 +384,12 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
     if (plane_open) close_plane();
     o << "    return f;\n}\n";
     // pixel culling (lensed_frame_cull): only when the model is the lens and
-    // the framed source alone, so a skipped pixel's value is exactly 0
-    o << "static inline bool cull_" << k << "(uint* data, float4 base, const float4* tab, float rho)\n{\n"
-      << "    (void)data; (void)base; (void)tab; (void)rho;\n";
-    if (model_culls(objs)) {
+    // the framed source alone, so a skipped pixel's value is exactly 0; the
+    // candidate's constants are formed once (cull_consts_k, in set_params)
+    const bool culls = model_culls(objs);
+    o << "static inline void cull_consts_" << k << "(uint* data, float4* cc)\n{\n"
+      << "    (void)data; (void)cc;\n";
+    if (culls) {
         const bool shear = pre_fn == "lensed_epl_pre_shear";
         o << "    const struct data_" << lm << "* s = " << data_ptr(lm, lens_off, "data") << ";\n"
           << "    struct data_" << sm << "* g = " << data_ptr(sm, src_off, "data") << ";\n"
@@ -399,11 +401,16 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
           << (shear ? "lensed_frob(float4(1.0f - s->shear.x, -s->shear.y, -s->shear.z, 1.0f - s->shear.w))"
                     : "1.41421356f")
           << ";\n"
-          << "    return lensed_frame_cull(s, base, tab, n0, n1, sqrtf(dot(t0, t0) + dot(t1, t1)), lnorm, frame_cutoff_"
-          << sm << "(g), rho);\n";
-    } else {
-        o << "    return false;\n";
+          << "    lensed_frame_cull_consts(s, n0, n1, sqrtf(dot(t0, t0) + dot(t1, t1)), lnorm, frame_cutoff_" << sm
+          << "(g), cc);\n";
     }
+    o << "}\n";
+    o << "static inline bool cull_" << k << "(uint* data, float4 base, const float4* tab, const float4* cc, float rho)\n{\n"
+      << "    (void)data; (void)base; (void)tab; (void)cc; (void)rho;\n";
+    if (culls)
+        o << "    return lensed_frame_cull(" << data_ptr(lm, lens_off, "data") << ", base, tab, cc, rho);\n";
+    else
+        o << "    return false;\n";
     o << "}\n";
     return o.str();
 }

@@ -19,6 +19,9 @@
 // rows are contiguous (16-byte stride), so lanes reading nearby rows hit
 // different LDS banks
 #define LENSED_EPL_AT(comp, i) ((comp) * LENSED_EPL_ROWS + (i))
+// per candidate in npre[]: the nq node pairs, then the pixel cull's constants
+#define LENSED_NPRE_CULL 3
+#define LENSED_NPRE_STRIDE(nq) ((nq) + LENSED_NPRE_CULL)
 
 struct lensed_desc {
     int width, height, npix, model;  // image size, pixels, model index
@@ -40,7 +43,7 @@ struct lensed_desc {
     int npart, table_mode, item_e, pad4;  // chi^2 partials per candidate (8-row x 64-column blocks); EPL table
                                           // in use; first emulator node item (NT + 1 nodes per candidate)
     long long psft_off;                  // floats into psf[]: the taps transposed, [psfw][psfh]
-    long long npre_off;                  // float4 into npre[]: the nodes' EPL lens-frame pairs ([ncand][nq])
+    long long npre_off;                  // float4 into npre[]: node pairs, cull constants ([ncand][nq + 3])
     float qrad;                          // largest distance of a quadrature node from the pixel centre
     int cull;                            // skip pixels whose rays provably all miss the source
 };

@@ -144,18 +144,30 @@ inline float lensed_epl_slope_bound(const D* self) {
 
 inline float lensed_frob(mat22 m) { return sqrtf(m.x * m.x + m.y * m.y + m.z * m.z + m.w * m.w); }
 
+// the candidate's constants, once (set_params): cc[0] = (n0, n1), cc[1] =
+// (|F|, |bwd| (|1-t| A + B) |T|, |I - S| |T|, cutoff + 1), cc[2].x = t
 template <class D>
-inline bool lensed_frame_cull(const D* self, float4 base, const float4* tab, float2 n0, float2 n1, float tnorm,
-                              float lnorm, float cutoff, float rho) {
-    const float2 us =
-        lensed_frame_trace(base, n0, n1, lensed_clamp_deflection(lensed_epl_table_vector_pre(self, base, tab)));
+inline void lensed_frame_cull_consts(const D* self, float2 n0, float2 n1, float tnorm, float lnorm, float cutoff,
+                                     float4* cc) {
+    cc[0] = float4(n0.x, n0.y, n1.x, n1.y);
+    cc[1] = float4(lensed_frob(self->fwd), lensed_frob(self->bwd) * lensed_epl_slope_bound(self) * tnorm,
+                   lnorm * tnorm, cutoff + 1.0f);
+    cc[2] = float4(1.0f - 2.0f * self->hexp, 0.0f, 0.0f, 0.0f);
+}
+
+// per pixel: whether all its rays provably miss the source
+template <class D>
+inline bool lensed_frame_cull(const D* self, float4 base, const float4* tab, const float4* cc, float rho) {
+    const float4 a = cc[0], b = cc[1];
+    const float t = cc[2].x;
+    const float2 us = lensed_frame_trace(base, float2(a.x, a.y), float2(a.z, a.w),
+                                         lensed_clamp_deflection(lensed_epl_table_vector_pre(self, base, tab)));
     const float rc = sqrtf(base.x * base.x + base.y * base.y);
-    const float fr = lensed_frob(self->fwd) * rho;
-    const float rlo = rc - fr, rhi = rc + fr;
-    const float t = 1.0f - 2.0f * self->hexp;
-    const float pw = fmaxf(exp2(-t * log2(rlo)), exp2(-t * log2(rhi)));  // sup of |u|^(-t)
-    const float dy = lnorm * rho + lensed_frob(self->bwd) * lensed_epl_slope_bound(self) * pw * fr;
-    return rlo > 0.0f && sqrtf(us.x * us.x + us.y * us.y) - tnorm * dy > cutoff + 1.0f;
+    const float fr = b.x * rho;
+    const float rlo = rc - fr;
+    const float pw = exp2(-t * log2(t > 0.0f ? rlo : rc + fr));  // sup of |u|^(-t) over [rlo, rc + fr]
+    const float d = rho * (b.z + b.y * pw * b.x);                 // |T| times the bound on |y - y_c|
+    return rlo > 0.0f && sqrtf(us.x * us.x + us.y * us.y) - d > b.w;
 }
 
 }  // namespace lcl

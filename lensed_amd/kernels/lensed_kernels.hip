@@ -45,10 +45,15 @@ __device__ inline void lensed_node_pairs(const lensed_desc& d, u32* blk, const f
                                          float4* __restrict__ npre, int cand)
 {
     const float4* q = quad + d.quad_off;
-    float4* out = npre + d.npre_off + (long long)cand * d.nq;
+    float4* out = npre + d.npre_off + (long long)cand * LENSED_NPRE_STRIDE(d.nq);
     for (int n = 0; n < d.nq; ++n) {
         lcl::float4 v = lcl::epl_pre_model<K>(blk, lcl::float2(q[n].x, q[n].y), false);
         out[n] = make_float4(v.x, v.y, v.z, v.w);
+    }
+    if (d.cull) {  // the candidate's constants of the pixel cull (lensed_frame_cull)
+        lcl::float4 cc[LENSED_NPRE_CULL];
+        lcl::cull_consts_model<K>(blk, cc);
+        for (int i = 0; i < LENSED_NPRE_CULL; ++i) out[d.nq + i] = make_float4(cc[i].x, cc[i].y, cc[i].z, cc[i].w);
     }
 }
 
@@ -115,7 +120,7 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int item, int ca
             // F (p - c) + F q_n: the lens-frame pair of each ray from the
             // pixel's part and the node's (from set_params, scalar loads)
             const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
-            const float4* np_c = npre + d.npre_off + (long long)cand * d.nq;
+            const float4* np_c = npre + d.npre_off + (long long)cand * LENSED_NPRE_STRIDE(d.nq);
             // waves whose pixels all provably miss the source (lensed_cull) skip
             // their rays: the pixels' values are 0 either way
             const int wave = (int)threadIdx.x / 64;
@@ -189,7 +194,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 template <int K>
 __device__ inline void lensed_cull_item(const lensed_desc& d, int item, int cand, int tile,
                                         const u32* __restrict__ blocks, const float4* __restrict__ tables,
-                                        float4* tab_lds, int* __restrict__ trace)
+                                        const float4* __restrict__ npre, float4* tab_lds, int* __restrict__ trace)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
@@ -205,7 +210,8 @@ __device__ inline void lensed_cull_item(const lensed_desc& d, int item, int cand
         const float cx = d.pcs[0] + d.pcs[2] * (float)px;
         const float cy = d.pcs[1] + d.pcs[3] * (float)py;
         const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
-        lit = !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, d.qrad);
+        const float4* cc = npre + d.npre_off + (long long)cand * LENSED_NPRE_STRIDE(d.nq) + d.nq;
+        lit = !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, (const lcl::float4*)cc, d.qrad);
     }
     const bool any = __any(lit);
     if ((threadIdx.x & 63) == 0) trace[(long long)item * (LENSED_RENDER_BS / 64) + threadIdx.x / 64] = any ? 1 : 0;
@@ -214,7 +220,8 @@ __device__ inline void lensed_cull_item(const lensed_desc& d, int item, int cand
 
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
-    const u32* __restrict__ blocks, const float4* __restrict__ tables, int* __restrict__ trace)
+    const u32* __restrict__ blocks, const float4* __restrict__ tables, const float4* __restrict__ npre,
+    int* __restrict__ trace)
 {
     extern __shared__ float4 tab_lds[];
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
@@ -227,7 +234,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         switch (d.model) {
 #define LENSED_CULL_CASE(K)                                                             \
     case K:                                                                             \
-        lensed_cull_item<K>(d, item, cand, tile, blocks, tables, tab_lds, trace);       \
+        lensed_cull_item<K>(d, item, cand, tile, blocks, tables, npre, tab_lds, trace); \
         break;
             LENSED_FOR_MODELS(LENSED_CULL_CASE)
 #undef LENSED_CULL_CASE

@@ -234,11 +234,13 @@ extern "C" void host_cull(const float* params, int w, int h, const float* q, int
     lcl::set_params_0(data, params);
     static lcl::float4 tab[LENSED_EPL_TABLE_F4];
     build_table(params[4], params[3], tab);
+    lcl::float4 cc[LENSED_NPRE_CULL];
+    lcl::cull_consts_0(data, cc);
     int skipped = 0, lit = 0;
     for (int k = 0; k < w * h; ++k) {
         const float cx = 1.0f + (float)(k % w), cy = 1.0f + (float)(k / w);
         const lcl::float4 b = lcl::epl_pre_0(data, lcl::float2(cx, cy), true);
-        if (!lcl::cull_0(data, b, tab, rho)) continue;
+        if (!lcl::cull_0(data, b, tab, cc, rho)) continue;
         ++skipped;
         for (int n = 0; n < nq; ++n) {
             const lcl::float4 d = lcl::epl_pre_0(data, lcl::float2(q[2 * n], q[2 * n + 1]), false);
