@@ -118,6 +118,7 @@ struct lensed {
     void* d_npre = nullptr;      // per candidate and node: EPL lens-frame pair of the node offset
     size_t cap_npre = 0;
     int* d_trace = nullptr;      // per render item and wave: trace its rays (lensed_cull)
+    int* d_live = nullptr;       // count, then the render items lensed_cull kept
     size_t cap_trace = 0;
     // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem
     hipEvent_t ev[9] = {};

@@ -3,7 +3,8 @@
 // a flat list of work items and find their system by the item ranges.
 #pragma once
 
-#define LENSED_RENDER_BS 256  // threads (= pixels) per render work item
+#define LENSED_RENDER_BS 256  // threads (= pixels) per render work item: a 16 x 16 block,
+#define LENSED_RENDER_SIDE 16 // an 8 x 8 square of it per wave
 #define LENSED_CONV_TX 64     // conv tile width  (threads along x)
 #define LENSED_CONV_TY 4      // threads along y
 #define LENSED_CONV_RY 8      // output rows per thread, large launches (a multiple of 8)

@@ -60,10 +60,11 @@ __device__ inline void lensed_node_pairs(const lensed_desc& d, u32* blk, const f
 extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const float* __restrict__ params, u32* __restrict__ blocks, const float4* __restrict__ quad,
-    float4* __restrict__ npre)
+    float4* __restrict__ npre, int* __restrict__ live)
 {
     int item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= nitems) return;
+    if (item == 0 && live) live[0] = 0;  // the cull pass's list of lit render items starts empty
     int s = lensed_find_sys<2>(sys, nsys, item);
     const lensed_desc& d = sys[s];
     int cand = item - d.item_s;
@@ -88,6 +89,19 @@ extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
 }
 
 // ------------------------------------------------------------------ render
+// Pixel of this thread in render item `tile`: items are 16 x 16 blocks, each
+// wave an 8 x 8 square of one (compact, so that waves lie wholly inside or
+// outside the region a source lights, for the pixel cull); false outside
+// the image.
+__device__ inline bool lensed_render_pixel(const lensed_desc& d, int tile, int& px, int& py) {
+    const int ntx = (d.width + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
+    const int by = tile / ntx, bx = tile - by * ntx;
+    const int w = (int)threadIdx.x >> 6, l = (int)threadIdx.x & 63;
+    px = bx * LENSED_RENDER_SIDE + (w & 1) * 8 + (l & 7);
+    py = by * LENSED_RENDER_SIDE + (w >> 1) * 8 + (l >> 3);
+    return px < d.width && py < d.height;
+}
+
 // One thread per pixel; the 2-D quadrature loop runs in the thread with the
 // node table in scalar registers and the candidate's object block held in
 // SGPRs (wave-uniform), so the per-ray work is pure VALU.
@@ -108,10 +122,9 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int item, int ca
         for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
         __syncthreads();
     }
-    int k = tile * LENSED_RENDER_BS + (int)threadIdx.x;
-    if (k < d.npix) {
-        int py = k / d.width;
-        int px = k - py * d.width;
+    int px, py;
+    if (lensed_render_pixel(d, tile, px, py)) {
+        const int k = py * d.width + px;
         // pixel centre as kernel/lensed.cl:25, then one ray per node
         const float cx = d.pcs[0] + d.pcs[2] * (float)px;
         const float cy = d.pcs[1] + d.pcs[3] * (float)py;
@@ -157,11 +170,14 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int item, int ca
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void lensed_render(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const u32* __restrict__ blocks, const float4* __restrict__ quad, const float4* __restrict__ tables,
-    const float4* __restrict__ npre, const int* __restrict__ trace, float* __restrict__ model,
-    float* __restrict__ errp)
+    const float4* __restrict__ npre, const int* __restrict__ trace, const int* __restrict__ live,
+    float* __restrict__ model, float* __restrict__ errp)
 {
     extern __shared__ float4 tab_lds[];
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    // after a cull pass, only the items it listed (live[1..live[0]])
+    const int n = live ? live[0] : nitems;
+    for (int j = blockIdx.x; j < n; j += gridDim.x) {
+        const int item = live ? live[1 + j] : j;
         int s = lensed_find_sys<0>(sys, nsys, item);
         const lensed_desc& d = sys[s];
         int local = item - d.item_r;
@@ -185,16 +201,20 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 }
 
 // -------------------------------------------------------------------- cull
-// Before the render: for each render item and wave, whether any of its
-// pixels can receive light (lensed_frame_cull in epl_table.h bounds where a
-// pixel's rays land).  Same items, grid and staged table as the render, so
-// the render only reads one flag per wave and keeps its registers for the
-// rays.  Only for systems with d.cull (a tabulated EPL lens and one framed
-// source with a cutoff).
+// Before the render: for each render item, whether any of its pixels can
+// receive light (lensed_frame_cull in epl_table.h bounds where a pixel's
+// rays land).  Items with no lit pixel get their zero model values here and
+// are left out of the render; the others go to a list that the render's
+// blocks share out evenly (lit items bunch in the middle of the image, so a
+// fixed item-to-block mapping would leave the blocks of the edge tiles idle),
+// with one flag per wave for the waves of a listed item that stay dark.
+// Only systems with d.cull (a tabulated EPL lens and one framed source with a
+// cutoff) are tested; the items of the others are all listed.
 template <int K>
-__device__ inline void lensed_cull_item(const lensed_desc& d, int item, int cand, int tile,
+__device__ inline bool lensed_cull_item(const lensed_desc& d, int item, int cand, int tile,
                                         const u32* __restrict__ blocks, const float4* __restrict__ tables,
-                                        const float4* __restrict__ npre, float4* tab_lds, int* __restrict__ trace)
+                                        const float4* __restrict__ npre, float4* tab_lds, int* __restrict__ trace,
+                                        float* __restrict__ model, float* __restrict__ errp)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
@@ -203,10 +223,10 @@ __device__ inline void lensed_cull_item(const lensed_desc& d, int item, int cand
     const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
     for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
     __syncthreads();
-    const int k = tile * LENSED_RENDER_BS + (int)threadIdx.x;
+    int px, py;
     bool lit = false;
-    if (k < d.npix) {
-        const int py = k / d.width, px = k - py * d.width;
+    const bool in = lensed_render_pixel(d, tile, px, py);
+    if (in) {
         const float cx = d.pcs[0] + d.pcs[2] * (float)px;
         const float cy = d.pcs[1] + d.pcs[3] * (float)py;
         const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
@@ -215,32 +235,63 @@ __device__ inline void lensed_cull_item(const lensed_desc& d, int item, int cand
     }
     const bool any = __any(lit);
     if ((threadIdx.x & 63) == 0) trace[(long long)item * (LENSED_RENDER_BS / 64) + threadIdx.x / 64] = any ? 1 : 0;
-    __syncthreads();  // the table is replaced by the next item
+    // also the barrier before the next item replaces the table
+    const bool keep = __syncthreads_or(lit) != 0;
+    if (!keep && in) {
+        const long long o = d.model_off + (long long)cand * d.npix + (long long)py * d.width + px;
+        model[o] = 0.0f;
+        if (errp) errp[o] = 0.0f;
+    }
+    return keep;
 }
 
+// live[0] = number of listed items (zeroed by lensed_set_params), live[1..]
+// their item indices.  A block gathers its lit items in LDS and reserves
+// room for a full chunk with one atomic, so the counter sees a few thousand
+// atomics per launch rather than one per item.
+#define LENSED_CULL_CHUNK 512
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const u32* __restrict__ blocks, const float4* __restrict__ tables, const float4* __restrict__ npre,
-    int* __restrict__ trace)
+    int* __restrict__ trace, int* __restrict__ live, float* __restrict__ model, float* __restrict__ errp)
 {
     extern __shared__ float4 tab_lds[];
+    __shared__ int chunk[LENSED_CULL_CHUNK];
+    __shared__ int at;
+    // the chunk's fill, the same in every thread (keep is block-uniform)
+    int n = 0;
+    auto flush = [&]() {
+        __syncthreads();
+        if (threadIdx.x == 0) at = atomicAdd(live, n);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += LENSED_RENDER_BS) live[1 + at + i] = chunk[i];
+        __syncthreads();  // before chunk and at are written again
+        n = 0;
+    };
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
         int s = lensed_find_sys<0>(sys, nsys, item);
         const lensed_desc& d = sys[s];
-        if (!d.cull) continue;
-        int local = item - d.item_r;
-        int cand = local / d.ntile_r;
-        int tile = local - cand * d.ntile_r;
-        switch (d.model) {
-#define LENSED_CULL_CASE(K)                                                             \
-    case K:                                                                             \
-        lensed_cull_item<K>(d, item, cand, tile, blocks, tables, npre, tab_lds, trace); \
+        bool keep = true;
+        if (d.cull) {
+            int local = item - d.item_r;
+            int cand = local / d.ntile_r;
+            int tile = local - cand * d.ntile_r;
+            switch (d.model) {
+#define LENSED_CULL_CASE(K)                                                                                \
+    case K:                                                                                                \
+        keep = lensed_cull_item<K>(d, item, cand, tile, blocks, tables, npre, tab_lds, trace, model, errp); \
         break;
-            LENSED_FOR_MODELS(LENSED_CULL_CASE)
+                LENSED_FOR_MODELS(LENSED_CULL_CASE)
 #undef LENSED_CULL_CASE
-            default: break;
+                default: break;
+            }
+        }
+        if (keep) {
+            if (threadIdx.x == 0) chunk[n] = item;
+            if (++n == LENSED_CULL_CHUNK) flush();
         }
     }
+    if (n > 0) flush();
 }
 
 // -------------------------------------------------------------- EPL tables
