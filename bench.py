@@ -32,8 +32,8 @@ sys.path.insert(0, ROOT)
 METRIC = "logL evals/sec at 256² px, batched+emulated vs plain GPU, |ΔlogL|≤0.01"
 # algorithmic work per evaluation of this workload, counted from the
 # reference formulas (add/mul/div = 1, FMA = 2, transcendental = 1):
-# 165 flops per ray x 65536 px x 49 rays, PSF 441 MAC per pixel, chi^2 4 flops
-FLOP_RENDER = 165 * 65536 * 49
+# 165 flops per ray traced (49 rays per pixel), PSF 441 MAC per pixel,
+# chi^2 4 flops
 FLOP_CONV = 2 * 441 * 65536
 FLOP_CHI2 = 4 * 65536
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X published FP32 (packed / matrix) peak
@@ -117,6 +117,9 @@ def time_plain(wl, n):
 
 
 def kernel_times(wl, K, launches, flags=0):
+    """Stage times (ms per launch, hipEvents on the launch stream), with the
+    render stage split into the cull pass and the render kernel, and what
+    the render traced (the same candidates every launch)."""
     from lensed_amd._lib import LENSED_EMULATE, LENSED_PROFILE
     from lensed_amd.lensed import Context
     emu = EMULATOR if flags & LENSED_EMULATE else None
@@ -124,11 +127,37 @@ def kernel_times(wl, K, launches, flags=0):
         cubes = wl.candidates(ctx.ndims(), K)
         ctx.loglike_batch(cubes)
         calls0, ms0 = ctx.profile()
+        cull0 = ctx.render_stats()[3]
         for _ in range(launches):
             ctx.loglike_batch(cubes)
         calls, ms = ctx.profile()
+        items, kept, waves, cull1 = ctx.render_stats()
     n = calls - calls0
-    return {k: (ms[k] - ms0[k]) / n for k in ms}
+    out = {k: (ms[k] - ms0[k]) / n for k in ms}
+    out["cull_pass"] = (cull1 - cull0) / n
+    out["render_kernel"] = out["render"] - out["cull_pass"]
+    return out, {"render_items": items, "items_kept_by_cull": kept, "waves_traced": waves,
+                 "rays_traced_frac": waves / (items * (256 // 64))}
+
+
+def render_roofline(kt, tr, K):
+    """The render kernel against its bound, VALU issue, over the rays it
+    traced (the pixel cull skips the rest, whose values are exactly 0)."""
+    rays = tr["waves_traced"] * 64 * 49
+    s = kt["render_kernel"] / 1e3
+    achieved = 165 * rays / s / 1e12
+    return {"bound": "valu", "kernel": "lensed_render", "achieved": round(achieved, 3), "peak": PEAK_FP32,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
+            "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED, 4),
+            "traffic": int(TRAFFIC_RENDER_4096 * K / 4096), "traffic_unit": "bytes per launch (rocprofv3 --pmc)",
+            "algorithmic_flop_per_launch": 165 * rays, "rays_traced_per_launch": rays,
+            "rays_traced_frac": round(tr["rays_traced_frac"], 4),
+            "avg_launch_ms": round(kt["render_kernel"], 4), "cull_pass_ms": round(kt["cull_pass"], 4),
+            "valu_instr_per_ray": RAY_VALU,
+            "valu_issue_frac": round(RAY_VALU * rays / 64 / VALU_ISSUE_PER_S / s, 4),
+            "note": "over the rays the kernel traces (165 flop each by the reference formulas, the "
+                    "tenth-order EPL series); the table path does that work in 35 VALU instructions per ray, "
+                    "so frac can exceed 1: valu_issue_frac is the kernel's position against its actual bound"}
 
 
 def near_truth(wl, pids, n, scale=2e-4, seed=7):
@@ -275,18 +304,14 @@ def main():
     wl, pids = make_workload(K)
     if args.value_only:
         dt, _, emu_stats = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_EMULATE)
-        kt = kernel_times(wl, K, 3, LENSED_EMULATE)
-        achieved = FLOP_RENDER * K / (kt["render"] / 1e3) / 1e12
+        kt, tr = kernel_times(wl, K, 3, LENSED_EMULATE)
         print(json.dumps({"metric": METRIC, "value": round(args.steps * K / dt, 2), "unit": "logL evals/s",
                           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                           "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True,
                           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
                           "config": {"workload": "C3 (see the full run)", "candidates_per_launch": K,
                                      "legs": "batched + emulated only"},
-                          "roofline": {"bound": "valu", "kernel": "lensed_render", "achieved": round(achieved, 3),
-                                       "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
-                                       "traffic": int(TRAFFIC_RENDER_4096 * K / 4096),
-                                       "avg_launch_ms": round(kt["render"], 4)},
+                          "roofline": render_roofline(kt, tr, K), "render_trace": tr,
                           "stages_ms_per_launch": {k: round(v, 4) for k, v in kt.items()},
                           "cpu_baseline": None}))
         return
@@ -302,10 +327,8 @@ def main():
     plain_rate, lplain = time_plain(wl, args.plain)
     # plain and batched use the same kernels: identical logL per candidate
     same = bool(np.array_equal(lplain, lnew[: len(lplain)]))
-    kt = kernel_times(wl, K, 3, LENSED_EMULATE)
-    kt_exact = kernel_times(wl, K, 2)
-    render_s = kt["render"] / 1e3
-    achieved = FLOP_RENDER * K / render_s / 1e12
+    kt, tr = kernel_times(wl, K, 3, LENSED_EMULATE)
+    kt_exact, _ = kernel_times(wl, K, 2)
     conv_s = kt["convolve"] / 1e3
     cpu = cpu_baseline(wl, pids, args.cpu_seconds)
     nest = None if args.no_nested else nested_compare(wl, pids, K)
@@ -344,19 +367,11 @@ def main():
                       "path": "one candidate per loglike() call, 4 kernels + 2 copies + sync",
                       "identical_logL_to_batched": same},
         "speedup_vs_plain": round(value / plain_rate, 3),
-        "roofline": {"bound": "valu", "kernel": "lensed_render", "achieved": round(achieved, 3),
-                     "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
-                     "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED, 4),
-                     "traffic": int(TRAFFIC_RENDER_4096 * K / 4096), "traffic_unit": "bytes per launch (rocprofv3 --pmc)",
-                     "algorithmic_flop_per_launch": FLOP_RENDER * K,
-                     "avg_launch_ms": round(kt["render"], 4),
-                     "valu_instr_per_ray": RAY_VALU,
-                     "valu_issue_frac": round(RAY_VALU * 65536 * 49 * K / 64 / VALU_ISSUE_PER_S / (kt["render"] / 1e3), 4),
-                     "note": "achieved counts the reference formulas (165 flop per ray, the tenth-order EPL series); "
-                             "the table path does that work in 35 VALU instructions per ray, so frac can exceed 1: "
-                             "valu_issue_frac is the kernel's position against its actual bound"},
+        "roofline": render_roofline(kt, tr, K),
+        "render_trace": tr,
         "stages_ms_per_launch": {k: round(v, 4) for k, v in kt.items()},
-        "stages_note": "set_params includes the emulator, held-out check and fallback table kernels",
+        "stages_note": "set_params includes the emulator, held-out check and fallback table kernels; render is "
+                       "cull_pass + render_kernel",
         "stages_ms_per_launch_exact": {k: round(v, 4) for k, v in kt_exact.items()},
         "conv_chi2": {"achieved_tflops": round((FLOP_CONV + FLOP_CHI2) * K / conv_s / 1e12, 3),
                       "avg_launch_ms": round(kt["convolve"], 4)},

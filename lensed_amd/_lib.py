@@ -115,6 +115,7 @@ SIGNATURES = {
     "lensed_emulator_eval": (c_int, [POINTER(c_float), POINTER(c_float), POINTER(c_float), c_int, c_int,
                                      POINTER(c_float), POINTER(c_float)]),
     "lensed_emulator_stats": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
+    "lensed_render_stats": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
     "lensed_fits_read": (c_int, [c_char_p, POINTER(c_size_t), POINTER(c_size_t), POINTER(POINTER(c_float)),
                                  POINTER(c_double)]),
     "lensed_fits_free": (None, [c_void_p]),

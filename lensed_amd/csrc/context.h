@@ -63,7 +63,7 @@ struct System {
 struct Layout {
     std::vector<lensed_desc> desc;  // systems with ncand > 0
     std::vector<int> sys_of_desc;
-    int n_set = 0, n_render = 0, n_conv = 0, n_final = 0, n_table = 0;
+    int n_set = 0, n_render = 0, n_cull = 0, n_conv = 0, n_final = 0, n_table = 0;
     int n_nodes = 0, n_tcands = 0;  // emulator: table nodes and candidates with a table
     size_t lds_conv = 0, lds_render = 0;
     bool conv_small = false;        // short tiles: launch too small to fill the GPU
@@ -108,6 +108,7 @@ struct lensed {
     float* d_errs = nullptr;
     float emu_tol = 4e-6f;
     const lsd::Layout* last_table_layout = nullptr;
+    const lsd::Layout* last_layout = nullptr;  // of the last launch (lensed_render_stats)
     // pinned host staging
     float* h_params = nullptr;
     double* h_logl = nullptr;
@@ -121,8 +122,9 @@ struct lensed {
     int* d_live = nullptr;       // count, then the render items lensed_cull kept
     size_t cap_trace = 0;
     // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem
-    hipEvent_t ev[9] = {};
+    hipEvent_t ev[10] = {};  // the eight stages' bounds, then the end of the cull pass
     double prof_ms[8] = {};
+    double prof_cull_ms = 0;  // the cull pass's share of the render stage
     long prof_calls = 0;
 };
 

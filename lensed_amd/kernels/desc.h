@@ -5,6 +5,7 @@
 
 #define LENSED_RENDER_BS 256  // threads (= pixels) per render work item: a 16 x 16 block,
 #define LENSED_RENDER_SIDE 16 // an 8 x 8 square of it per wave
+#define LENSED_CULL_GROUP 16  // render items per pixel-cull work item (one staged EPL table)
 #define LENSED_CONV_TX 64     // conv tile width  (threads along x)
 #define LENSED_CONV_TY 4      // threads along y
 #define LENSED_CONV_RY 8      // output rows per thread, large launches (a multiple of 8)
@@ -41,8 +42,9 @@ struct lensed_desc {
     int item_t, epl_obj, epl_q, epl_t;   // table items; EPL object word offset, q and t param index
     int quad_off, nq, conv_ry, sep_rank;  // nodes (float4 offset, count); conv rows/thread; PSF rank (0 = direct)
     long long sep_off;                   // floats into psf[]: per rank, column [psfh] then row [psfw] factor
-    int npart, table_mode, item_e, pad4;  // chi^2 partials per candidate (8-row x 64-column blocks); EPL table
-                                          // in use; first emulator node item (NT + 1 nodes per candidate)
+    int npart, table_mode, item_e, item_k;  // chi^2 partials per candidate (8-row x 64-column blocks); EPL table
+                                            // in use; first emulator node item (NT + 1 nodes per candidate);
+                                            // first cull item (LENSED_CULL_GROUP render items each)
     long long psft_off;                  // floats into psf[]: the taps transposed, [psfw][psfh]
     long long npre_off;                  // float4 into npre[]: node pairs, cull constants ([ncand][nq + 3])
     float qrad;                          // largest distance of a quadrature node from the pixel centre

@@ -29,7 +29,7 @@ __device__ inline int lensed_find_sys(const lensed_desc* __restrict__ sys, int n
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         int first = WHICH == 0 ? sys[mid].item_r : WHICH == 1 ? sys[mid].item_c : WHICH == 2 ? sys[mid].item_s
-                  : WHICH == 3 ? sys[mid].item_f : sys[mid].item_t;
+                  : WHICH == 3 ? sys[mid].item_f : WHICH == 4 ? sys[mid].item_t : sys[mid].item_k;
         if (first <= item) lo = mid; else hi = mid - 1;
     }
     return lo;
@@ -208,13 +208,15 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 // blocks share out evenly (lit items bunch in the middle of the image, so a
 // fixed item-to-block mapping would leave the blocks of the edge tiles idle),
 // with one flag per wave for the waves of a listed item that stay dark.
+// A cull work item is LENSED_CULL_GROUP render items of one candidate: the
+// table is staged once for all of them and the waves meet at one barrier.
 // Only systems with d.cull (a tabulated EPL lens and one framed source with a
 // cutoff) are tested; the items of the others are all listed.
 template <int K>
-__device__ inline bool lensed_cull_item(const lensed_desc& d, int item, int cand, int tile,
-                                        const u32* __restrict__ blocks, const float4* __restrict__ tables,
-                                        const float4* __restrict__ npre, float4* tab_lds, int* __restrict__ trace,
-                                        float* __restrict__ model, float* __restrict__ errp)
+__device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0, int ng,
+                                         const u32* __restrict__ blocks, const float4* __restrict__ tables,
+                                         const float4* __restrict__ npre, float4* tab_lds, int* wave_lit,
+                                         int* __restrict__ trace)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
@@ -223,33 +225,33 @@ __device__ inline bool lensed_cull_item(const lensed_desc& d, int item, int cand
     const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
     for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
     __syncthreads();
-    int px, py;
-    bool lit = false;
-    const bool in = lensed_render_pixel(d, tile, px, py);
-    if (in) {
-        const float cx = d.pcs[0] + d.pcs[2] * (float)px;
-        const float cy = d.pcs[1] + d.pcs[3] * (float)py;
-        const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
-        const float4* cc = npre + d.npre_off + (long long)cand * LENSED_NPRE_STRIDE(d.nq) + d.nq;
-        lit = !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, (const lcl::float4*)cc, d.qrad);
+    const float4* cc = npre + d.npre_off + (long long)cand * LENSED_NPRE_STRIDE(d.nq) + d.nq;
+    const int wave = (int)threadIdx.x >> 6;
+    for (int t = 0; t < ng; ++t) {
+        int px, py;
+        bool lit = false;
+        if (lensed_render_pixel(d, g0 + t, px, py)) {
+            const float cx = d.pcs[0] + d.pcs[2] * (float)px;
+            const float cy = d.pcs[1] + d.pcs[3] * (float)py;
+            const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
+            lit = !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, (const lcl::float4*)cc, d.qrad);
+        }
+        const int any = __any(lit) ? 1 : 0;
+        if ((threadIdx.x & 63) == 0) {
+            const long long item = d.item_r + (long long)cand * d.ntile_r + g0 + t;
+            trace[item * (LENSED_RENDER_BS / 64) + wave] = any;
+            wave_lit[t * (LENSED_RENDER_BS / 64) + wave] = any;
+        }
     }
-    const bool any = __any(lit);
-    if ((threadIdx.x & 63) == 0) trace[(long long)item * (LENSED_RENDER_BS / 64) + threadIdx.x / 64] = any ? 1 : 0;
-    // also the barrier before the next item replaces the table
-    const bool keep = __syncthreads_or(lit) != 0;
-    if (!keep && in) {
-        const long long o = d.model_off + (long long)cand * d.npix + (long long)py * d.width + px;
-        model[o] = 0.0f;
-        if (errp) errp[o] = 0.0f;
-    }
-    return keep;
+    __syncthreads();  // wave_lit complete; the table may be replaced
 }
 
 // live[0] = number of listed items (zeroed by lensed_set_params), live[1..]
-// their item indices.  A block gathers its lit items in LDS and reserves
+// their item indices.  A block gathers its listed items in LDS and reserves
 // room for a full chunk with one atomic, so the counter sees a few thousand
 // atomics per launch rather than one per item.
 #define LENSED_CULL_CHUNK 512
+static_assert(LENSED_RENDER_BS == 256, "a render item is four waves");
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const u32* __restrict__ blocks, const float4* __restrict__ tables, const float4* __restrict__ npre,
@@ -257,8 +259,10 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
 {
     extern __shared__ float4 tab_lds[];
     __shared__ int chunk[LENSED_CULL_CHUNK];
+    __shared__ int wave_lit[LENSED_CULL_GROUP * (LENSED_RENDER_BS / 64)];
     __shared__ int at;
-    // the chunk's fill, the same in every thread (keep is block-uniform)
+    // the chunk's fill, the same in every thread (what is listed is
+    // block-uniform)
     int n = 0;
     auto flush = [&]() {
         __syncthreads();
@@ -268,27 +272,45 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         __syncthreads();  // before chunk and at are written again
         n = 0;
     };
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-        int s = lensed_find_sys<0>(sys, nsys, item);
+    auto list = [&](int item) {
+        if (threadIdx.x == 0) chunk[n] = item;
+        if (++n == LENSED_CULL_CHUNK) flush();
+    };
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        int s = lensed_find_sys<5>(sys, nsys, w);
         const lensed_desc& d = sys[s];
-        bool keep = true;
-        if (d.cull) {
-            int local = item - d.item_r;
-            int cand = local / d.ntile_r;
-            int tile = local - cand * d.ntile_r;
-            switch (d.model) {
-#define LENSED_CULL_CASE(K)                                                                                \
-    case K:                                                                                                \
-        keep = lensed_cull_item<K>(d, item, cand, tile, blocks, tables, npre, tab_lds, trace, model, errp); \
-        break;
-                LENSED_FOR_MODELS(LENSED_CULL_CASE)
-#undef LENSED_CULL_CASE
-                default: break;
-            }
+        const int groups = (d.ntile_r + LENSED_CULL_GROUP - 1) / LENSED_CULL_GROUP;
+        const int local = w - d.item_k;
+        const int cand = local / groups;
+        const int g0 = (local - cand * groups) * LENSED_CULL_GROUP;
+        const int ng = min(LENSED_CULL_GROUP, d.ntile_r - g0);
+        const int item0 = d.item_r + cand * d.ntile_r + g0;
+        if (!d.cull) {
+            for (int t = 0; t < ng; ++t) list(item0 + t);
+            continue;
         }
-        if (keep) {
-            if (threadIdx.x == 0) chunk[n] = item;
-            if (++n == LENSED_CULL_CHUNK) flush();
+        switch (d.model) {
+#define LENSED_CULL_CASE(K)                                                                        \
+    case K:                                                                                        \
+        lensed_cull_group<K>(d, cand, g0, ng, blocks, tables, npre, tab_lds, wave_lit, trace);     \
+        break;
+            LENSED_FOR_MODELS(LENSED_CULL_CASE)
+#undef LENSED_CULL_CASE
+            default: break;
+        }
+        // list the lit items; the dark ones get their zeros here
+        for (int t = 0; t < ng; ++t) {
+            const int* f = wave_lit + t * (LENSED_RENDER_BS / 64);
+            if (f[0] | f[1] | f[2] | f[3]) {
+                list(item0 + t);
+            } else {
+                int px, py;
+                if (lensed_render_pixel(d, g0 + t, px, py)) {
+                    const long long o = d.model_off + (long long)cand * d.npix + (long long)py * d.width + px;
+                    model[o] = 0.0f;
+                    if (errp) errp[o] = 0.0f;
+                }
+            }
         }
     }
     if (n > 0) flush();

@@ -270,6 +270,15 @@ class Context:
         _check(L.load().lensed_emulator_stats(self.ptr, byref(nf), byref(nc), byref(mx)), "emulator_stats")
         return nf.value, nc.value, mx.value
 
+    def render_stats(self):
+        """(render items, items the pixel cull kept, waves traced) of the last
+        launch, and the cull pass's accumulated ms under LENSED_PROFILE."""
+        from ctypes import c_long
+        n, kept, waves, ms = c_long(), c_long(), c_long(), c_double()
+        _check(L.load().lensed_render_stats(self.ptr, byref(n), byref(kept), byref(waves), byref(ms)),
+               "render_stats")
+        return n.value, kept.value, waves.value, ms.value
+
     def profile(self):
         """Accumulated stage times (ms) with the reference's stage names."""
         ms = (c_double * 8)()

@@ -166,3 +166,31 @@ def test_c5_table_mode_near_truth(c5):
         print(f"C5 candidate {i}: series {le[i] - lref:+.2e}, table {lt[i] - lref:+.2e} vs FP64", flush=True)
         assert abs(le[i] - lref) <= 1e-2 and abs(lt[i] - lref) <= 1e-2, (i, le[i] - lref, lt[i] - lref)
     assert np.max(np.abs(lt - le)) <= 1e-2, np.max(np.abs(lt - le))
+
+
+def prior_draws(wl, n, seed):
+    rng = np.random.default_rng(seed)
+    return np.array([[rng.uniform(*wl.prior_range(k)) for k in pids_of(wl)] for _ in range(n)])
+
+
+def test_c4_culling_is_exact(c4):
+    """The pixel cull in a launch that mixes tested systems (tabulated EPL +
+    Sersic) with systems it never tests (SIE, listed whole): every system's
+    logL bit for bit that of tracing every ray, over prior draws.  A Sersic
+    profile of these indices reaches exact zero only tens of effective
+    radii out, beyond these images, so here every item stays listed."""
+    from lensed_amd._lib import LENSED_NO_CULL
+    wls = c4
+    phys = [prior_draws(wl, 32, 200 + s) for s, wl in enumerate(wls)]
+    out, stats = [], []
+    for extra in (0, LENSED_NO_CULL):
+        with Context([wl.spec() for wl in wls], max_batch=32, flags=LENSED_TABLE | LENSED_GRAPH | extra) as ctx:
+            out.append(ctx.loglike_phys(phys))
+            stats.append(ctx.render_stats()[:3])
+    for s in range(len(wls)):
+        assert np.all(np.isfinite(out[0][s])), s
+        assert np.array_equal(out[0][s], out[1][s]), s
+    items = sum(32 * (wl.shape[0] // 16) * (wl.shape[1] // 16) for wl in wls)
+    assert stats[1] == (items, items, 4 * items)
+    sie = sum(32 * (wl.shape[0] // 16) * (wl.shape[1] // 16) for wl in wls if wl.objects[0][1] == "sie")
+    assert stats[0][0] == items and sie <= stats[0][1] <= items, (stats, sie)

@@ -349,10 +349,11 @@ def test_c3_culling_is_exact(c3):
     edge[:, pids.index("lens.x")] = 128.0
     edge[:, pids.index("lens.y")] = 128.0
     edge[:, pids.index("lens.t")] = [0.6, 0.999, 1.0, 1.4]
-    res, render_ms = [], []
+    res, render_ms, stats = [], [], []
     for extra in (0, LENSED_NO_CULL):
         with Context(wl.spec(), max_batch=1024, flags=LENSED_TABLE | LENSED_GRAPH | extra) as ctx:
             lc, _ = ctx.loglike_batch(cubes)
+            stats.append(ctx.render_stats()[:3])
             ln = ctx.loglike_phys(np.concatenate([phys, edge]))
             planes = [ctx.render(p) for p in (edge[0], edge[2], edge[3], phys[0])]
             res.append((lc, ln, planes))
@@ -366,5 +367,49 @@ def test_c3_culling_is_exact(c3):
     for a, b in zip(pl, pl0):
         for k in a:
             assert np.array_equal(a[k], b[k]), k
-    print(f"render ms per launch: culled {render_ms[0]:.3f}, every ray {render_ms[1]:.3f}", flush=True)
+    print(f"render ms per launch: culled {render_ms[0]:.3f}, every ray {render_ms[1]:.3f}; "
+          f"(items, kept, waves traced) {stats}", flush=True)
     assert render_ms[0] < render_ms[1], render_ms
+    # 16 x 16 pixel items of four waves; without culling every wave of
+    # every item is traced, with it a strict subset over prior draws
+    items = 1024 * 256
+    assert stats[1] == (items, items, 4 * items)
+    assert stats[0][0] == items and 0 < stats[0][1] < items and stats[0][1] <= stats[0][2] <= 4 * stats[0][1]
+
+
+@pytest.mark.parametrize("shape", [(37, 53), (70, 33)])
+def test_culling_ragged_image_is_exact(shape):
+    """Images whose sides are not multiples of the 16-pixel render blocks:
+    the edge blocks' pixels outside the image are neither traced nor
+    written, and culled launches give bit for bit the logL and planes of
+    tracing every ray."""
+    from lensed_amd._lib import LENSED_NO_CULL, LENSED_TABLE
+    h, w = shape
+    cx, cy = (w + 1) / 2, (h + 1) / 2
+    objs = [("lens", "epl_plus_shear"), ("source", "gauss")]
+    pri = {"lens.x": f"unif {cx - 1} {cx + 1}", "lens.y": f"unif {cy - 1} {cy + 1}", "lens.r": "unif 4 8",
+           "lens.t": "unif 0.5 1.5", "lens.q": "unif 0.3 1", "lens.pa": "unif 0 180", "lens.g1": "unif -0.1 0.1",
+           "lens.g2": "unif -0.1 0.1", "source.x": f"unif {cx - 3} {cx + 3}", "source.y": f"unif {cy - 3} {cy + 3}",
+           "source.sigma": "unif 0.3 0.8", "source.mag": "unif -5 5", "source.q": "unif 0.4 1",
+           "source.pa": "unif 0 180"}
+    rng = np.random.default_rng(h * w)
+    image = rng.normal(0, 0.05, shape).astype(np.float32)
+    weight = np.full(shape, 400.0, np.float32)
+    psf = W.gaussian_psf(5, 2.0)
+    cubes = rng.random((256, 14))
+    res, stats = [], []
+    for extra in (0, LENSED_NO_CULL):
+        with Context(SystemSpec(objs, pri, image, weight, psf=psf), max_batch=256,
+                     flags=LENSED_TABLE | LENSED_GRAPH | extra) as ctx:
+            lc, phys = ctx.loglike_batch(cubes)
+            stats.append(ctx.render_stats()[:3])
+            p = np.zeros(14)
+            p[ctx.pmap()] = phys[0]
+            res.append((lc, ctx.render(p)))
+    (lc, pl), (lc0, pl0) = res
+    assert np.all(np.isfinite(lc)) and np.array_equal(lc, lc0)
+    for k in pl:
+        assert np.array_equal(pl[k], pl0[k]), k
+    items = 256 * ((h + 15) // 16) * ((w + 15) // 16)
+    assert stats[1] == (items, items, 4 * items)
+    assert stats[0][0] == items and 0 < stats[0][1] < items, stats
