@@ -215,8 +215,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 template <int K>
 __device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0, int ng,
                                          const u32* __restrict__ blocks, const float4* __restrict__ tables,
-                                         const float4* __restrict__ npre, float4* tab_lds, int* wave_lit,
-                                         int* __restrict__ trace)
+                                         const float4* __restrict__ npre, float4* tab_lds, int* wave_lit)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
@@ -224,24 +223,44 @@ __device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0,
     for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = i < d.dwords ? __builtin_amdgcn_readfirstlane(gb[i]) : 0u;
     const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
     for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
+    const float4* ccg = npre + d.npre_off + (long long)cand * LENSED_NPRE_STRIDE(d.nq) + d.nq;
+    lcl::float4 cc[LENSED_NPRE_CULL];
+#pragma unroll
+    for (int i = 0; i < LENSED_NPRE_CULL; ++i) cc[i] = lcl::float4(ccg[i].x, ccg[i].y, ccg[i].z, ccg[i].w);
+    const float ox = d.pcs[0], oy = d.pcs[1], sx = d.pcs[2], sy = d.pcs[3], qrad = d.qrad;
+    const int ntx = (d.width + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
     __syncthreads();
-    const float4* cc = npre + d.npre_off + (long long)cand * LENSED_NPRE_STRIDE(d.nq) + d.nq;
+    // coarse pass, one lane per 8 x 8 wave square of the group: the bound at
+    // the square's centre with rho grown by the square's half-diagonal holds
+    // for every ray of the square, so a square found dark here needs no
+    // per-pixel test (squares past the image edge are covered too)
+    bool dark_sq = false;
+    {
+        const int lane = (int)threadIdx.x & 63, t = lane >> 2, w = lane & 3;
+        if (t < ng) {
+            const int tile = g0 + t, by = tile / ntx, bx = tile - by * ntx;
+            const float px = (float)(bx * LENSED_RENDER_SIDE + (w & 1) * 8) + 3.5f;
+            const float py = (float)(by * LENSED_RENDER_SIDE + (w >> 1) * 8) + 3.5f;
+            const float rho = 3.5f * sqrtf(sx * sx + sy * sy) + qrad;
+            const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(ox + sx * px, oy + sy * py), true);
+            dark_sq = lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, cc, rho);
+        }
+    }
+    const unsigned long long dark = __ballot(dark_sq);
     const int wave = (int)threadIdx.x >> 6;
     for (int t = 0; t < ng; ++t) {
-        int px, py;
-        bool lit = false;
-        if (lensed_render_pixel(d, g0 + t, px, py)) {
-            const float cx = d.pcs[0] + d.pcs[2] * (float)px;
-            const float cy = d.pcs[1] + d.pcs[3] * (float)py;
-            const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
-            lit = !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, (const lcl::float4*)cc, d.qrad);
+        int any = 0;
+        if (!((dark >> (t * 4 + wave)) & 1ull)) {
+            int px, py;
+            bool lit = false;
+            if (lensed_render_pixel(d, g0 + t, px, py)) {
+                const float cx = ox + sx * (float)px, cy = oy + sy * (float)py;
+                const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
+                lit = !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, cc, qrad);
+            }
+            any = __any(lit) ? 1 : 0;
         }
-        const int any = __any(lit) ? 1 : 0;
-        if ((threadIdx.x & 63) == 0) {
-            const long long item = d.item_r + (long long)cand * d.ntile_r + g0 + t;
-            trace[item * (LENSED_RENDER_BS / 64) + wave] = any;
-            wave_lit[t * (LENSED_RENDER_BS / 64) + wave] = any;
-        }
+        if ((threadIdx.x & 63) == 0) wave_lit[t * (LENSED_RENDER_BS / 64) + wave] = any;
     }
     __syncthreads();  // wave_lit complete; the table may be replaced
 }
@@ -290,22 +309,29 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
             continue;
         }
         switch (d.model) {
-#define LENSED_CULL_CASE(K)                                                                        \
-    case K:                                                                                        \
-        lensed_cull_group<K>(d, cand, g0, ng, blocks, tables, npre, tab_lds, wave_lit, trace);     \
+#define LENSED_CULL_CASE(K)                                                                 \
+    case K:                                                                                 \
+        lensed_cull_group<K>(d, cand, g0, ng, blocks, tables, npre, tab_lds, wave_lit);     \
         break;
             LENSED_FOR_MODELS(LENSED_CULL_CASE)
 #undef LENSED_CULL_CASE
             default: break;
         }
-        // list the lit items; the dark ones get their zeros here
+        // the render's per-wave flags of the group's items, one coalesced store
+        if ((int)threadIdx.x < ng * (LENSED_RENDER_BS / 64))
+            trace[(long long)item0 * (LENSED_RENDER_BS / 64) + threadIdx.x] = wave_lit[threadIdx.x];
+        // list the lit items; the dark ones get their zeros here, a row of
+        // 16 pixels per 16 threads
+        const int ntx = (d.width + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
         for (int t = 0; t < ng; ++t) {
             const int* f = wave_lit + t * (LENSED_RENDER_BS / 64);
             if (f[0] | f[1] | f[2] | f[3]) {
                 list(item0 + t);
             } else {
-                int px, py;
-                if (lensed_render_pixel(d, g0 + t, px, py)) {
+                const int tile = g0 + t, by = tile / ntx, bx = tile - by * ntx;
+                const int px = bx * LENSED_RENDER_SIDE + ((int)threadIdx.x & 15);
+                const int py = by * LENSED_RENDER_SIDE + ((int)threadIdx.x >> 4);
+                if (px < d.width && py < d.height) {
                     const long long o = d.model_off + (long long)cand * d.npix + (long long)py * d.width + px;
                     model[o] = 0.0f;
                     if (errp) errp[o] = 0.0f;
