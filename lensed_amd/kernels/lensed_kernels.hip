@@ -569,7 +569,8 @@ __device__ inline void lensed_conv_item(const lensed_desc& d, int cand, int t,
                                         const float* __restrict__ model, const float* __restrict__ image,
                                         const float* __restrict__ weight, const float* __restrict__ psf,
                                         double* __restrict__ partials, float* __restrict__ conv_out,
-                                        float* __restrict__ chi2_out, float* tile, double* red, float* rows8)
+                                        float* __restrict__ chi2_out, const int* __restrict__ trace, float* tile,
+                                        double* red, float* rows8)
 {
 #pragma clang fp reassociate(off)
     constexpr int TX = LENSED_CONV_TX, TH = LENSED_CONV_TY * RY;
@@ -599,7 +600,30 @@ __device__ inline void lensed_conv_item(const lensed_desc& d, int cand, int t,
             }
         }
     };
-    if (d.psfw > 0) {
+    // after a pixel cull: if every render item under the tile's input
+    // region (the tile and its PSF halo, clamped to the image as the tile
+    // load clamps) was found dark, the model there is exactly 0 and so is
+    // the convolution: chi^2 needs only data and weights
+    bool dark = false;
+    if (trace && d.cull && d.psfw > 0) {
+        const int ox = tx0 - d.psfw / 2, oy = ty0 - d.psfh / 2;
+        const int c0 = min(max(ox, 0), W - 1), c1 = min(max(ox + TX + d.psfw - 2, 0), W - 1);
+        const int r0 = min(max(oy, 0), H - 1), r1 = min(max(oy + TH + d.psfh - 2, 0), H - 1);
+        const int ntx = (W + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
+        const int bx0 = c0 / LENSED_RENDER_SIDE, nbx = c1 / LENSED_RENDER_SIDE - bx0 + 1;
+        const int by0 = r0 / LENSED_RENDER_SIDE, nby = r1 / LENSED_RENDER_SIDE - by0 + 1;
+        const int4* flags = (const int4*)trace + d.item_r + (long long)cand * d.ntile_r;
+        int lit = 0;
+        for (int i = threadIdx.x; i < nbx * nby; i += LENSED_CONV_BS) {
+            const int by = by0 + i / nbx, bx = bx0 + i % nbx;
+            const int4 f = flags[by * ntx + bx];
+            lit |= f.x | f.y | f.z | f.w;
+        }
+        dark = __syncthreads_or(lit) == 0;
+    }
+    if (dark) {
+        fetch();
+    } else if (d.psfw > 0) {
         lensed_conv_load_tile<RY>(d, m, tile, tx0, ty0, lx, ly, d.psfw, d.psfh);
         __syncthreads();
         fetch();
@@ -667,7 +691,7 @@ __device__ inline void lensed_conv_chi2_body(const lensed_desc* __restrict__ sys
                                              const float* __restrict__ model, const float* __restrict__ image,
                                              const float* __restrict__ weight, const float* __restrict__ psf,
                                              double* __restrict__ partials, float* __restrict__ conv_out,
-                                             float* __restrict__ chi2_out)
+                                             float* __restrict__ chi2_out, const int* __restrict__ trace)
 {
     extern __shared__ float4 tile4[];
     float* tile = (float*)tile4;
@@ -676,10 +700,13 @@ __device__ inline void lensed_conv_chi2_body(const lensed_desc* __restrict__ sys
     auto run = [&](int item) {
         int s = lensed_find_sys<1>(sys, nsys, item);
         const lensed_desc& d = sys[s];
+        // candidate-fastest order: a persistent block's items spread over
+        // the image's tiles (dark tiles after a pixel cull cost less)
         int local = item - d.item_c;
-        int cand = local / d.ntile_c;
-        int t = local - cand * d.ntile_c;
-        lensed_conv_item<RY>(d, cand, t, model, image, weight, psf, partials, conv_out, chi2_out, tile, red, rows8);
+        int t = local / d.ncand;
+        int cand = local - t * d.ncand;
+        lensed_conv_item<RY>(d, cand, t, model, image, weight, psf, partials, conv_out, chi2_out, trace, tile, red,
+                             rows8);
     };
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) run(item);
 }
@@ -688,20 +715,22 @@ extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) void lensed_conv_chi2(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const float* __restrict__ model, const float* __restrict__ image,
     const float* __restrict__ weight, const float* __restrict__ psf,
-    double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out)
+    double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out,
+    const int* __restrict__ trace)
 {
     lensed_conv_chi2_body<LENSED_CONV_RY>(sys, nsys, nitems, model, image, weight, psf, partials, conv_out,
-                                          chi2_out);
+                                          chi2_out, trace);
 }
 
 extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) void lensed_conv_chi2_small(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const float* __restrict__ model, const float* __restrict__ image,
     const float* __restrict__ weight, const float* __restrict__ psf,
-    double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out)
+    double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out,
+    const int* __restrict__ trace)
 {
     lensed_conv_chi2_body<LENSED_CONV_RY_SMALL>(sys, nsys, nitems, model, image, weight, psf, partials,
-                                                conv_out, chi2_out);
+                                                conv_out, chi2_out, trace);
 }
 
 // ---------------------------------------------------------------- finalize
