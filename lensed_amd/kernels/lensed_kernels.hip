@@ -35,6 +35,25 @@ __device__ inline int lensed_find_sys(const lensed_desc* __restrict__ sys, int n
     return lo;
 }
 
+// Work of a persistent grid split with the XCDs in mind: workgroups go to
+// the eight XCDs in turn (block b on XCD b % 8) and each XCD has its own L2,
+// so each XCD takes a contiguous eighth of the work [first, end) in steps of
+// its own block count; neighbouring items (one candidate's, sharing its EPL
+// table) then meet in one L2.
+__device__ inline void lensed_xcd_split(int n, int& first, int& end, int& step) {
+    const int g = (int)gridDim.x, b = (int)blockIdx.x;
+    if (g < 8) {
+        first = b;
+        end = n;
+        step = g;
+        return;
+    }
+    const int x = b & 7;
+    first = (int)((long long)n * x / 8) + (b >> 3);
+    end = (int)((long long)n * (x + 1) / 8);
+    step = (g - x + 7) >> 3;  // blocks on XCD x
+}
+
 // --------------------------------------------------------------- set_params
 // One thread per candidate: derived object constants from the physical
 // parameters, written as a data block of LENSED_DWORDS words; with an EPL
@@ -175,8 +194,9 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 {
     extern __shared__ float4 tab_lds[];
     // after a cull pass, only the items it listed (live[1..live[0]])
-    const int n = live ? live[0] : nitems;
-    for (int j = blockIdx.x; j < n; j += gridDim.x) {
+    int first, end, step;
+    lensed_xcd_split(live ? live[0] : nitems, first, end, step);
+    for (int j = first; j < end; j += step) {
         const int item = live ? live[1 + j] : j;
         int s = lensed_find_sys<0>(sys, nsys, item);
         const lensed_desc& d = sys[s];
@@ -295,7 +315,9 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         if (threadIdx.x == 0) chunk[n] = item;
         if (++n == LENSED_CULL_CHUNK) flush();
     };
-    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    int first, end, step;
+    lensed_xcd_split(nitems, first, end, step);
+    for (int w = first; w < end; w += step) {
         int s = lensed_find_sys<5>(sys, nsys, w);
         const lensed_desc& d = sys[s];
         const int groups = (d.ntile_r + LENSED_CULL_GROUP - 1) / LENSED_CULL_GROUP;
