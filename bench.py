@@ -312,6 +312,8 @@ def main():
                           "config": {"workload": "C3 (see the full run)", "candidates_per_launch": K,
                                      "legs": "batched + emulated only"},
                           "roofline": render_roofline(kt, tr, K), "render_trace": tr,
+                          "emulator": {"fallbacks": emu_stats[0], "checked": emu_stats[1],
+                                       "max_heldout_abs_error_vs_fp64_series": emu_stats[2]},
                           "stages_ms_per_launch": {k: round(v, 4) for k, v in kt.items()},
                           "cpu_baseline": None}))
         return

@@ -290,98 +290,108 @@ __device__ inline float silu_d(float z, float& dsilu) {
     return z * s;
 }
 
-// one hidden 128 x 128 layer in MFMA, primal and tangent (no bias on the tangent)
-__device__ inline void mfma_layer(const bf16x4* __restrict__ wl, const float* __restrict__ bias, const bf16x4 (&hb)[16],
-                                  const bf16x4 (&tb)[16], float (&z)[4][16], float (&dz)[4][16], int lane) {
-    f32x16 acc[4], tac[4];
+// Output rows of an MFMA 32x32 tile held by a lane: element v of M-tile mt
+// is unit 32 mt + 8 (v / 4) + 4 (lane / 32) + v % 4 of sample lane % 32.  The
+// four elements 4 gq .. 4 gq + 3 of M-tile mt are exactly what the lane holds
+// of the next layer's B operand at K step 4 mt + gq, so a layer's output is
+// packed into the next layer's operands one M-tile at a time, and only the
+// operands (never a whole layer of FP32 activations) stay live: the kernel
+// then fits several waves per SIMD to hide the weight loads.
+__device__ inline int mlp_unit(int mt, int v, int lane) { return 32 * mt + 8 * (v / 4) + 4 * (lane / 32) + (v % 4); }
+
+// activations z (primal) and dz (tangent) of M-tile mt -> silu, packed as
+// the bf16 B operands hb / tb at K steps 4 mt .. 4 mt + 3; h and dh keep the
+// FP32 values for a following FP32 layer
+__device__ inline void pack_tile(const float (&z)[16], const float (&dz)[16], int mt, bf16x4 (&hb)[16],
+                                 bf16x4 (&tb)[16], float (&h)[16], float (&dh)[16]) {
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-        for (int v = 0; v < 16; ++v) acc[mt][v] = tac[mt][v] = 0.0f;
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            bf16x4 a = wl[(mt * 16 + s) * 64 + lane];
-            acc[mt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, hb[s], acc[mt], 0, 0, 0);
-            tac[mt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, tb[s], tac[mt], 0, 0, 0);
-        }
+    for (int v = 0; v < 16; ++v) {
+        float ds;
+        h[v] = silu_d(z[v], ds);
+        dh[v] = ds * dz[v];
     }
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+    for (int gq = 0; gq < 4; ++gq) {
+        bf16x4 a, b;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            int o = 32 * mt + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-            z[mt][v] = acc[mt][v] + bias[o];
-            dz[mt][v] = tac[mt][v];
+        for (int i = 0; i < 4; ++i) {
+            a[i] = f2bf(h[4 * gq + i]);
+            b[i] = f2bf(dh[4 * gq + i]);
         }
+        hb[4 * mt + gq] = a;
+        tb[4 * mt + gq] = b;
+    }
 }
 
-// activations (z -> silu) of a layer as the bf16 B operands of the next,
-// with the tangent silu'(z) dz
-__device__ inline void to_operands(const float (&z)[4][16], const float (&dz)[4][16], bf16x4 (&hb)[16],
-                                   bf16x4 (&tb)[16], float (&h)[4][16], float (&dh)[4][16]) {
+// M-tile mt of a hidden 128 x 128 layer in MFMA, primal and tangent (no bias
+// on the tangent)
+__device__ inline void mfma_tile(const bf16x4* __restrict__ wl, const float* __restrict__ bias, const bf16x4 (&hb)[16],
+                                 const bf16x4 (&tb)[16], int mt, float (&z)[16], float (&dz)[16], int lane) {
+    f32x16 acc, tac;
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+    for (int v = 0; v < 16; ++v) acc[v] = tac[v] = 0.0f;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            float ds;
-            h[mt][v] = silu_d(z[mt][v], ds);
-            dh[mt][v] = ds * dz[mt][v];
-        }
+    for (int s = 0; s < 16; ++s) {
+        bf16x4 a = wl[(mt * 16 + s) * 64 + lane];
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, hb[s], acc, 0, 0, 0);
+        tac = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, tb[s], tac, 0, 0, 0);
+    }
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-        for (int gq = 0; gq < 4; ++gq) {
-            bf16x4 a, b;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                a[i] = f2bf(h[mt][4 * gq + i]);
-                b[i] = f2bf(dh[mt][4 * gq + i]);
-            }
-            hb[4 * mt + gq] = a;
-            tb[4 * mt + gq] = b;
-        }
+    for (int v = 0; v < 16; ++v) {
+        z[v] = acc[v] + bias[mlp_unit(mt, v, lane)];
+        dz[v] = tac[v];
+    }
 }
 
 // the network on 32 samples per wave (lanes l and l + 32 share sample l % 32):
 // output y and its derivative dy along the input direction dx
 __device__ inline void mlp_wave(const float* __restrict__ ws, const bf16x4* __restrict__ w2, const bf16x4* __restrict__ w3,
                                 const float (&x)[4], const float (&dx)[4], float (&y)[2], float (&dy)[2], int lane) {
-    float z[4][16], dz[4][16], h[4][16], dh[4][16];
-    // layer 1 in FP32 (K = 4)
+    bf16x4 hb[16], tb[16], nb[16], nt[16];
+    float z[16], dz[16], h[16], dh[16];
+    // layer 1 in FP32 (K = 4), straight into layer 2's operands
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+    for (int mt = 0; mt < 4; ++mt) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            int o = 32 * mt + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+            const int o = mlp_unit(mt, v, lane);
             const float* wr = ws + kW1 + o * kIn;
-            z[mt][v] = ws[kB1 + o] + wr[0] * x[0] + wr[1] * x[1] + wr[2] * x[2] + wr[3] * x[3];
-            dz[mt][v] = wr[0] * dx[0] + wr[1] * dx[1] + wr[2] * dx[2] + wr[3] * dx[3];
+            z[v] = ws[kB1 + o] + wr[0] * x[0] + wr[1] * x[1] + wr[2] * x[2] + wr[3] * x[3];
+            dz[v] = wr[0] * dx[0] + wr[1] * dx[1] + wr[2] * dx[2] + wr[3] * dx[3];
         }
-    bf16x4 hb[16], tb[16];
-    to_operands(z, dz, hb, tb, h, dh);
-    mfma_layer(w2, ws + kB2, hb, tb, z, dz, lane);
-    to_operands(z, dz, hb, tb, h, dh);
-    mfma_layer(w3, ws + kB3, hb, tb, z, dz, lane);
-    to_operands(z, dz, hb, tb, h, dh);
-    // layer 4 in FP32: each lane holds 64 of the 128 inputs of its sample
+        pack_tile(z, dz, mt, hb, tb, h, dh);
+    }
+    // layer 2 in MFMA, into layer 3's operands
 #pragma unroll
-    for (int c = 0; c < kOut; ++c) {
-        float acc = 0.0f, tac = 0.0f;
+    for (int mt = 0; mt < 4; ++mt) {
+        mfma_tile(w2, ws + kB2, hb, tb, mt, z, dz, lane);
+        pack_tile(z, dz, mt, nb, nt, h, dh);
+    }
+    // layer 3 in MFMA; layer 4 in FP32 on its tiles as they come (each lane
+    // holds 64 of the 128 inputs of its sample; summed over M-tiles, then
+    // units, as one pass over the layer would)
+    float acc[kOut], tac[kOut];
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
+    for (int c = 0; c < kOut; ++c) acc[c] = tac[c] = 0.0f;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        mfma_tile(w3, ws + kB3, nb, nt, mt, z, dz, lane);
+        pack_tile(z, dz, mt, hb, tb, h, dh);
+#pragma unroll
+        for (int c = 0; c < kOut; ++c)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                int o = 32 * mt + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-                float wv = ws[kW4 + c * kHidden + o];
-                acc += wv * h[mt][v];
-                tac += wv * dh[mt][v];
+                const float wv = ws[kW4 + c * kHidden + mlp_unit(mt, v, lane)];
+                acc[c] += wv * h[v];
+                tac[c] += wv * dh[v];
             }
-        acc += __shfl_xor(acc, 32, 64);
-        tac += __shfl_xor(tac, 32, 64);
-        y[c] = acc + ws[kB4 + c];
-        dy[c] = tac;
+    }
+#pragma unroll
+    for (int c = 0; c < kOut; ++c) {
+        acc[c] += __shfl_xor(acc[c], 32, 64);
+        tac[c] += __shfl_xor(tac[c], 32, 64);
+        y[c] = acc[c] + ws[kB4 + c];
+        dy[c] = tac[c];
     }
 }
 
@@ -423,7 +433,9 @@ __device__ inline int find_sys_t(const lensed_desc* __restrict__ sys, int nsys, 
 
 constexpr int kNodes = LENSED_EPL_ROWS;  // per candidate
 
-__global__ __launch_bounds__(256) void k_emu_nodes(const lensed_desc* __restrict__ sys, int nsys, int nitems,
+// two waves per SIMD (at most 256 VGPRs): one wave's weight loads hide
+// behind the other's MFMA
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_emu_nodes(const lensed_desc* __restrict__ sys, int nsys, int nitems,
                                                    const float* __restrict__ params, const float* __restrict__ wf,
                                                    const bf16x4* __restrict__ wp, float4* __restrict__ nodes) {
     const int lane = threadIdx.x & 63;
