@@ -310,16 +310,16 @@ __device__ inline void pack_tile(const float (&z)[16], const float (&dz)[16], in
         h[v] = silu_d(z[v], ds);
         dh[v] = ds * dz[v];
     }
+    // round to nearest even, as f2bf: gfx950's packed conversion, two
+    // values per instruction
+    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+    typedef float fl4 __attribute__((ext_vector_type(4)));
 #pragma unroll
     for (int gq = 0; gq < 4; ++gq) {
-        bf16x4 a, b;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            a[i] = f2bf(h[4 * gq + i]);
-            b[i] = f2bf(dh[4 * gq + i]);
-        }
-        hb[4 * mt + gq] = a;
-        tb[4 * mt + gq] = b;
+        const fl4 a = {h[4 * gq], h[4 * gq + 1], h[4 * gq + 2], h[4 * gq + 3]};
+        const fl4 b = {dh[4 * gq], dh[4 * gq + 1], dh[4 * gq + 2], dh[4 * gq + 3]};
+        hb[4 * mt + gq] = __builtin_bit_cast(bf16x4, __builtin_convertvector(a, bf4));
+        tb[4 * mt + gq] = __builtin_bit_cast(bf16x4, __builtin_convertvector(b, bf4));
     }
 }
 
