@@ -45,10 +45,11 @@ PEAK_HBM = 8000.0  # GB/s
 # of the generated program (rocprofv3 SQ_INSTS_VALU per launch agrees)
 RAY_VALU = 35
 VALU_ISSUE_PER_S = 256 * 2.4e9  # wave instructions per second
-# HBM bytes per 4096-candidate render launch, measured with rocprofv3 --pmc:
-# FETCH_SIZE 165455 KiB (x 2, the factor a 1 GiB copy calibrates) in and
-# WRITE_SIZE 1048576 KiB (x 1) out, i.e. the 4096 model images
-TRAFFIC_RENDER_4096 = (165455 * 2 + 1048576) * 1024
+# HBM bytes per 4096-candidate render launch (prior draws, pixel cull on),
+# measured with rocprofv3 --pmc: FETCH_SIZE 154900 KiB (x 2, the factor a
+# 1 GiB copy calibrates) in and WRITE_SIZE 280300 KiB (x 1) out, the lit
+# 16 x 16 blocks of the model images (the cull pass writes the dark ones)
+TRAFFIC_RENDER_4096 = (154900 * 2 + 280300) * 1024
 
 
 def make_workload(K):
