@@ -108,14 +108,14 @@ extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
 }
 
 // ------------------------------------------------------------------ render
-// Pixel of this thread in render item `tile`: items are 16 x 16 blocks, each
-// wave an 8 x 8 square of one (compact, so that waves lie wholly inside or
-// outside the region a source lights, for the pixel cull); false outside
-// the image.
-__device__ inline bool lensed_render_pixel(const lensed_desc& d, int tile, int& px, int& py) {
+// Pixel of this thread in 8 x 8 square w (0..3) of render item `tile`:
+// items are 16 x 16 blocks (compact squares, so that a wave's pixels lie
+// wholly inside or outside the region a source lights, for the pixel cull);
+// false outside the image.
+__device__ inline bool lensed_render_pixel(const lensed_desc& d, int tile, int w, int& px, int& py) {
     const int ntx = (d.width + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
     const int by = tile / ntx, bx = tile - by * ntx;
-    const int w = (int)threadIdx.x >> 6, l = (int)threadIdx.x & 63;
+    const int l = (int)threadIdx.x & 63;
     px = bx * LENSED_RENDER_SIDE + (w & 1) * 8 + (l & 7);
     py = by * LENSED_RENDER_SIDE + (w >> 1) * 8 + (l >> 3);
     return px < d.width && py < d.height;
@@ -125,11 +125,10 @@ __device__ inline bool lensed_render_pixel(const lensed_desc& d, int tile, int& 
 // node table in scalar registers and the candidate's object block held in
 // SGPRs (wave-uniform), so the per-ray work is pure VALU.
 template <int K, bool ERR>
-__device__ inline void lensed_render_item(const lensed_desc& d, int item, int cand, int tile,
+__device__ inline void lensed_render_item(const lensed_desc& d, int cand, int tile, int sq, bool on,
                                           const u32* __restrict__ blocks, const float4* __restrict__ quad,
                                           const float4* __restrict__ tables, const float4* __restrict__ npre,
-                                          const int* __restrict__ trace, float4* tab_lds,
-                                          float* __restrict__ model, float* __restrict__ errp)
+                                          float4* tab_lds, float* __restrict__ model, float* __restrict__ errp)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
@@ -142,7 +141,7 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int item, int ca
         __syncthreads();
     }
     int px, py;
-    if (lensed_render_pixel(d, tile, px, py)) {
+    if (on && lensed_render_pixel(d, tile, sq, px, py)) {
         const int k = py * d.width + px;
         // pixel centre as kernel/lensed.cl:25, then one ray per node
         const float cx = d.pcs[0] + d.pcs[2] * (float)px;
@@ -153,20 +152,14 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int item, int ca
             // pixel's part and the node's (from set_params, scalar loads)
             const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
             const float4* np_c = npre + d.npre_off + (long long)cand * LENSED_NPRE_STRIDE(d.nq);
-            // waves whose pixels all provably miss the source (lensed_cull) skip
-            // their rays: the pixels' values are 0 either way
-            const int wave = (int)threadIdx.x / 64;
-            const bool go = !d.cull || __builtin_amdgcn_readfirstlane(trace[(long long)item * (LENSED_RENDER_BS / 64) + wave]);
-            if (go) {
-                for (int n = 0; n < d.nq; ++n) {
-                    const float4 node = q[n];
-                    const float4 np = np_c[n];
-                    const lcl::float4 pre(base.x + np.x, base.y + np.y, base.z + np.z, base.w + np.w);
-                    float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y),
-                                                    (const lcl::float4*)tab_lds, pre);
-                    f += node.z * v;
-                    if (ERR) e += node.w * v;
-                }
+            for (int n = 0; n < d.nq; ++n) {
+                const float4 node = q[n];
+                const float4 np = np_c[n];
+                const lcl::float4 pre(base.x + np.x, base.y + np.y, base.z + np.z, base.w + np.w);
+                float v = lcl::compute_model<K>(blk, lcl::float2(cx + node.x, cy + node.y),
+                                                (const lcl::float4*)tab_lds, pre);
+                f += node.z * v;
+                if (ERR) e += node.w * v;
             }
         } else {
             for (int n = 0; n < d.nq; ++n) {
@@ -189,29 +182,38 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int item, int ca
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void lensed_render(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const u32* __restrict__ blocks, const float4* __restrict__ quad, const float4* __restrict__ tables,
-    const float4* __restrict__ npre, const int* __restrict__ trace, const int* __restrict__ live,
-    float* __restrict__ model, float* __restrict__ errp)
+    const float4* __restrict__ npre, const int* __restrict__ live, float* __restrict__ model,
+    float* __restrict__ errp)
 {
     extern __shared__ float4 tab_lds[];
-    // after a cull pass, only the items it listed (live[1..live[0]])
+    // after a cull pass, only the lit 8 x 8 squares it listed, four to an
+    // entry, all four of one candidate (live[1..live[0]]: square = item * 4 +
+    // w, -1 for none); otherwise every item, square w to wave w
+    const int wave = (int)threadIdx.x >> 6;
     int first, end, step;
-    lensed_xcd_split(live ? live[0] : nitems, first, end, step);
+    lensed_xcd_split(live ? live[0] / 4 : nitems, first, end, step);
     for (int j = first; j < end; j += step) {
-        const int item = live ? live[1 + j] : j;
+        int item = j, mine = j, sq = wave;
+        bool on = true;
+        if (live) {
+            item = live[1 + 4 * j] >> 2;  // the entry's candidate is that of its first square
+            const int v = __builtin_amdgcn_readfirstlane(live[1 + 4 * j + wave]);
+            on = v >= 0;
+            mine = on ? v >> 2 : item;
+            sq = on ? v & 3 : 0;
+        }
         int s = lensed_find_sys<0>(sys, nsys, item);
         const lensed_desc& d = sys[s];
         int local = item - d.item_r;
         int cand = local / d.ntile_r;
-        int tile = local - cand * d.ntile_r;
+        int tile = mine - d.item_r - cand * d.ntile_r;
         switch (d.model) {
-#define LENSED_RENDER_CASE(K)                                                                            \
-    case K:                                                                                              \
-        if (errp)                                                                                        \
-            lensed_render_item<K, true>(d, item, cand, tile, blocks, quad, tables, npre, trace, tab_lds, model, \
-                                        errp);                                                           \
-        else                                                                                             \
-            lensed_render_item<K, false>(d, item, cand, tile, blocks, quad, tables, npre, trace, tab_lds, model, \
-                                         errp);                                                          \
+#define LENSED_RENDER_CASE(K)                                                                                   \
+    case K:                                                                                                     \
+        if (errp)                                                                                               \
+            lensed_render_item<K, true>(d, cand, tile, sq, on, blocks, quad, tables, npre, tab_lds, model, errp);  \
+        else                                                                                                    \
+            lensed_render_item<K, false>(d, cand, tile, sq, on, blocks, quad, tables, npre, tab_lds, model, errp); \
         break;
             LENSED_FOR_MODELS(LENSED_RENDER_CASE)
 #undef LENSED_RENDER_CASE
@@ -273,7 +275,7 @@ __device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0,
         if (!((dark >> (t * 4 + wave)) & 1ull)) {
             int px, py;
             bool lit = false;
-            if (lensed_render_pixel(d, g0 + t, px, py)) {
+            if (lensed_render_pixel(d, g0 + t, wave, px, py)) {
                 const float cx = ox + sx * (float)px, cy = oy + sy * (float)py;
                 const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
                 lit = !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, cc, qrad);
@@ -285,11 +287,14 @@ __device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0,
     __syncthreads();  // wave_lit complete; the table may be replaced
 }
 
-// live[0] = number of listed items (zeroed by lensed_set_params), live[1..]
-// their item indices.  A block gathers its listed items in LDS and reserves
-// room for a full chunk with one atomic, so the counter sees a few thousand
-// atomics per launch rather than one per item.
-#define LENSED_CULL_CHUNK 512
+// live[0] = number of listed squares (zeroed by lensed_set_params), then the
+// squares (item * 4 + w, -1 pads an entry), four to an entry, all four of one
+// candidate: the render gives a block one entry and each wave one square, so
+// no wave of the render idles on a dark square next to a lit one.  A block
+// gathers its entries in LDS and reserves room for a full chunk with one
+// atomic, so the counter sees a few thousand atomics per launch rather than
+// one per entry.
+#define LENSED_CULL_CHUNK 2048
 static_assert(LENSED_RENDER_BS == 256, "a render item is four waves");
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
@@ -311,9 +316,21 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         __syncthreads();  // before chunk and at are written again
         n = 0;
     };
-    auto list = [&](int item) {
-        if (threadIdx.x == 0) chunk[n] = item;
-        if (++n == LENSED_CULL_CHUNK) flush();
+    // a group lists at most 64 squares (16 items x 4), placed at once: lane
+    // i of the first wave stands for square i of the group; every wave forms
+    // the same ballot of the lit ones, so all agree on the count, and the
+    // group's last entry is padded with -1
+    static_assert(LENSED_CULL_GROUP * 4 == 64 && LENSED_CULL_CHUNK % 64 == 0, "a group's squares in one wave");
+    const int lane = (int)threadIdx.x & 63;
+    auto list_group = [&](int square0, bool lit) {
+        if (n + 64 > LENSED_CULL_CHUNK) flush();
+        const unsigned long long mask = __ballot(lit);
+        const int total = __popcll(mask), padded = (total + 3) & ~3;
+        if (threadIdx.x < 64) {
+            if (lit) chunk[n + __popcll(mask & ((1ull << lane) - 1ull))] = square0 + lane;
+            if (lane >= total && lane < padded) chunk[n + lane] = -1;
+        }
+        n += padded;
     };
     int first, end, step;
     lensed_xcd_split(nitems, first, end, step);
@@ -327,7 +344,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         const int ng = min(LENSED_CULL_GROUP, d.ntile_r - g0);
         const int item0 = d.item_r + cand * d.ntile_r + g0;
         if (!d.cull) {
-            for (int t = 0; t < ng; ++t) list(item0 + t);
+            list_group(4 * item0, lane < 4 * ng);
             continue;
         }
         switch (d.model) {
@@ -342,22 +359,26 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         // the render's per-wave flags of the group's items, one coalesced store
         if ((int)threadIdx.x < ng * (LENSED_RENDER_BS / 64))
             trace[(long long)item0 * (LENSED_RENDER_BS / 64) + threadIdx.x] = wave_lit[threadIdx.x];
-        // list the lit items; the dark ones get their zeros here, a row of
-        // 16 pixels per 16 threads
+        // list the lit squares; the dark ones get their zeros here: a dark
+        // item in rows of 16 pixels per 16 threads, a dark square of a lit
+        // item by its own wave
         const int ntx = (d.width + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
+        const int wave = (int)threadIdx.x >> 6;
+        list_group(4 * item0, lane < 4 * ng && wave_lit[lane] != 0);
         for (int t = 0; t < ng; ++t) {
             const int* f = wave_lit + t * (LENSED_RENDER_BS / 64);
+            int px = -1, py = -1;
             if (f[0] | f[1] | f[2] | f[3]) {
-                list(item0 + t);
+                if (!f[wave] && !lensed_render_pixel(d, g0 + t, wave, px, py)) px = -1;
             } else {
                 const int tile = g0 + t, by = tile / ntx, bx = tile - by * ntx;
-                const int px = bx * LENSED_RENDER_SIDE + ((int)threadIdx.x & 15);
-                const int py = by * LENSED_RENDER_SIDE + ((int)threadIdx.x >> 4);
-                if (px < d.width && py < d.height) {
-                    const long long o = d.model_off + (long long)cand * d.npix + (long long)py * d.width + px;
-                    model[o] = 0.0f;
-                    if (errp) errp[o] = 0.0f;
-                }
+                px = bx * LENSED_RENDER_SIDE + ((int)threadIdx.x & 15);
+                py = by * LENSED_RENDER_SIDE + ((int)threadIdx.x >> 4);
+            }
+            if (px >= 0 && px < d.width && py < d.height) {
+                const long long o = d.model_off + (long long)cand * d.npix + (long long)py * d.width + px;
+                model[o] = 0.0f;
+                if (errp) errp[o] = 0.0f;
             }
         }
     }

@@ -377,12 +377,12 @@ def test_c3_culling_is_exact(c3):
     assert stats[0][0] == items and 0 < stats[0][1] < items and stats[0][1] <= stats[0][2] <= 4 * stats[0][1]
 
 
-@pytest.mark.parametrize("shape", [(37, 53), (70, 33)])
-def test_culling_ragged_image_is_exact(shape):
+@pytest.mark.parametrize("shape,psf_size", [((37, 53), 5), ((70, 33), 5), ((48, 40), 0)])
+def test_culling_ragged_image_is_exact(shape, psf_size):
     """Images whose sides are not multiples of the 16-pixel render blocks:
     the edge blocks' pixels outside the image are neither traced nor
     written, and culled launches give bit for bit the logL and planes of
-    tracing every ray."""
+    tracing every ray, with and without a PSF."""
     from lensed_amd._lib import LENSED_NO_CULL, LENSED_TABLE
     h, w = shape
     cx, cy = (w + 1) / 2, (h + 1) / 2
@@ -395,7 +395,7 @@ def test_culling_ragged_image_is_exact(shape):
     rng = np.random.default_rng(h * w)
     image = rng.normal(0, 0.05, shape).astype(np.float32)
     weight = np.full(shape, 400.0, np.float32)
-    psf = W.gaussian_psf(5, 2.0)
+    psf = W.gaussian_psf(psf_size, 2.0) if psf_size else None
     cubes = rng.random((256, 14))
     res, stats = [], []
     for extra in (0, LENSED_NO_CULL):
