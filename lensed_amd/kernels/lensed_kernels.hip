@@ -54,6 +54,23 @@ __device__ inline void lensed_xcd_split(int n, int& first, int& end, int& step) 
     step = (g - x + 7) >> 3;  // blocks on XCD x
 }
 
+// The same eighth per XCD, cut into one contiguous run per block [first,
+// end): consecutive items then go to one block, which keeps what they share
+// (the render's staged table) instead of each block fetching it again.
+__device__ inline void lensed_xcd_run(int n, int& first, int& end) {
+    const int g = (int)gridDim.x, b = (int)blockIdx.x;
+    int lo = 0, hi = n, nb = g, k = b;
+    if (g >= 8) {
+        const int x = b & 7;
+        lo = (int)((long long)n * x / 8);
+        hi = (int)((long long)n * (x + 1) / 8);
+        nb = (g - x + 7) >> 3;  // blocks on XCD x
+        k = b >> 3;
+    }
+    first = lo + (int)((long long)(hi - lo) * k / nb);
+    end = lo + (int)((long long)(hi - lo) * (k + 1) / nb);
+}
+
 // --------------------------------------------------------------- set_params
 // One thread per candidate: derived object constants from the physical
 // parameters, written as a data block of LENSED_DWORDS words; with an EPL
@@ -125,7 +142,7 @@ __device__ inline bool lensed_render_pixel(const lensed_desc& d, int tile, int w
 // node table in scalar registers and the candidate's object block held in
 // SGPRs (wave-uniform), so the per-ray work is pure VALU.
 template <int K, bool ERR>
-__device__ inline void lensed_render_item(const lensed_desc& d, int cand, int tile, int sq, bool on,
+__device__ inline void lensed_render_item(const lensed_desc& d, int cand, int tile, int sq, bool on, bool restage,
                                           const u32* __restrict__ blocks, const float4* __restrict__ quad,
                                           const float4* __restrict__ tables, const float4* __restrict__ npre,
                                           float4* tab_lds, float* __restrict__ model, float* __restrict__ errp)
@@ -135,7 +152,8 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
 #pragma unroll
     for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = i < d.dwords ? __builtin_amdgcn_readfirstlane(gb[i]) : 0u;
     const float4* q = quad + d.quad_off;
-    if (d.table_mode) {  // the candidate's EPL angular table, staged in LDS
+    if (d.table_mode && restage) {  // the candidate's EPL angular table, staged in LDS
+        __syncthreads();  // the previous candidate's table is no longer read
         const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
         for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
         __syncthreads();
@@ -174,7 +192,6 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
         model[o] = f;
         if (ERR) errp[o] = e;
     }
-    if (d.table_mode) __syncthreads();  // the table is replaced by the next item
 }
 
 // eight waves per SIMD (at most 64 VGPRs): the ray loop's latencies
@@ -190,9 +207,10 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
     // entry, all four of one candidate (live[1..live[0]]: square = item * 4 +
     // w, -1 for none); otherwise every item, square w to wave w
     const int wave = (int)threadIdx.x >> 6;
-    int first, end, step;
-    lensed_xcd_split(live ? live[0] / 4 : nitems, first, end, step);
-    for (int j = first; j < end; j += step) {
+    int first, end;
+    lensed_xcd_run(live ? live[0] / 4 : nitems, first, end);
+    int staged = -1;  // first item of the candidate whose table is in LDS
+    for (int j = first; j < end; ++j) {
         int item = j, mine = j, sq = wave;
         bool on = true;
         if (live) {
@@ -207,13 +225,18 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
         int local = item - d.item_r;
         int cand = local / d.ntile_r;
         int tile = mine - d.item_r - cand * d.ntile_r;
+        const int key = d.item_r + cand * d.ntile_r;  // block-uniform
+        const bool restage = key != staged;
+        staged = key;
         switch (d.model) {
-#define LENSED_RENDER_CASE(K)                                                                                   \
-    case K:                                                                                                     \
-        if (errp)                                                                                               \
-            lensed_render_item<K, true>(d, cand, tile, sq, on, blocks, quad, tables, npre, tab_lds, model, errp);  \
-        else                                                                                                    \
-            lensed_render_item<K, false>(d, cand, tile, sq, on, blocks, quad, tables, npre, tab_lds, model, errp); \
+#define LENSED_RENDER_CASE(K)                                                                                \
+    case K:                                                                                                  \
+        if (errp)                                                                                            \
+            lensed_render_item<K, true>(d, cand, tile, sq, on, restage, blocks, quad, tables, npre, tab_lds, \
+                                        model, errp);                                                        \
+        else                                                                                                 \
+            lensed_render_item<K, false>(d, cand, tile, sq, on, restage, blocks, quad, tables, npre, tab_lds, \
+                                         model, errp);                                                       \
         break;
             LENSED_FOR_MODELS(LENSED_RENDER_CASE)
 #undef LENSED_RENDER_CASE
