@@ -37,26 +37,11 @@ __device__ inline int lensed_find_sys(const lensed_desc* __restrict__ sys, int n
 
 // Work of a persistent grid split with the XCDs in mind: workgroups go to
 // the eight XCDs in turn (block b on XCD b % 8) and each XCD has its own L2,
-// so each XCD takes a contiguous eighth of the work [first, end) in steps of
-// its own block count; neighbouring items (one candidate's, sharing its EPL
-// table) then meet in one L2.
-__device__ inline void lensed_xcd_split(int n, int& first, int& end, int& step) {
-    const int g = (int)gridDim.x, b = (int)blockIdx.x;
-    if (g < 8) {
-        first = b;
-        end = n;
-        step = g;
-        return;
-    }
-    const int x = b & 7;
-    first = (int)((long long)n * x / 8) + (b >> 3);
-    end = (int)((long long)n * (x + 1) / 8);
-    step = (g - x + 7) >> 3;  // blocks on XCD x
-}
-
-// The same eighth per XCD, cut into one contiguous run per block [first,
-// end): consecutive items then go to one block, which keeps what they share
-// (the render's staged table) instead of each block fetching it again.
+// so each XCD takes a contiguous eighth of the work, cut into one
+// contiguous run per block [first, end): neighbouring items (one
+// candidate's, sharing its EPL table) then meet in one L2, and consecutive
+// items go to one block, which keeps what they share (the staged table)
+// instead of each block fetching it again.
 __device__ inline void lensed_xcd_run(int n, int& first, int& end) {
     const int g = (int)gridDim.x, b = (int)blockIdx.x;
     int lo = 0, hi = n, nb = g, k = b;
@@ -258,7 +243,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 // Only systems with d.cull (a tabulated EPL lens and one framed source with a
 // cutoff) are tested; the items of the others are all listed.
 template <int K>
-__device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0, int ng,
+__device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0, int ng, bool restage,
                                          const u32* __restrict__ blocks, const float4* __restrict__ tables,
                                          const float4* __restrict__ npre, float4* tab_lds, int* wave_lit)
 {
@@ -266,15 +251,17 @@ __device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0,
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
 #pragma unroll
     for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = i < d.dwords ? __builtin_amdgcn_readfirstlane(gb[i]) : 0u;
-    const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
-    for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
+    if (restage) {  // a new candidate (the previous group's barrier has retired the old table)
+        const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
+        for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
+    }
     const float4* ccg = npre + d.npre_off + (long long)cand * LENSED_NPRE_STRIDE(d.nq) + d.nq;
     lcl::float4 cc[LENSED_NPRE_CULL];
 #pragma unroll
     for (int i = 0; i < LENSED_NPRE_CULL; ++i) cc[i] = lcl::float4(ccg[i].x, ccg[i].y, ccg[i].z, ccg[i].w);
     const float ox = d.pcs[0], oy = d.pcs[1], sx = d.pcs[2], sy = d.pcs[3], qrad = d.qrad;
     const int ntx = (d.width + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
-    __syncthreads();
+    __syncthreads();  // the table is staged; the previous group's wave_lit is read
     // coarse pass, one lane per 8 x 8 wave square of the group: the bound at
     // the square's centre with rho grown by the square's half-diagonal holds
     // for every ray of the square, so a square found dark here needs no
@@ -341,23 +328,31 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     };
     // a group lists at most 64 squares (16 items x 4), placed at once: lane
     // i of the first wave stands for square i of the group; every wave forms
-    // the same ballot of the lit ones, so all agree on the count, and the
-    // group's last entry is padded with -1
-    static_assert(LENSED_CULL_GROUP * 4 == 64 && LENSED_CULL_CHUNK % 64 == 0, "a group's squares in one wave");
+    // the same ballot of the lit ones, so all agree on the count.  A block
+    // takes consecutive groups, so one candidate's groups follow each other:
+    // an entry continues across them and is padded with -1 only where the
+    // candidate changes and before a flush, so the chunk only ever holds
+    // whole entries
+    static_assert(LENSED_CULL_GROUP * 4 == 64 && LENSED_CULL_CHUNK % 4 == 0, "a group's squares in one wave");
     const int lane = (int)threadIdx.x & 63;
-    auto list_group = [&](int square0, bool lit) {
-        if (n + 64 > LENSED_CULL_CHUNK) flush();
-        const unsigned long long mask = __ballot(lit);
-        const int total = __popcll(mask), padded = (total + 3) & ~3;
-        if (threadIdx.x < 64) {
-            if (lit) chunk[n + __popcll(mask & ((1ull << lane) - 1ull))] = square0 + lane;
-            if (lane >= total && lane < padded) chunk[n + lane] = -1;
-        }
-        n += padded;
+    auto pad = [&]() {
+        const int p = (4 - (n & 3)) & 3;
+        if ((int)threadIdx.x < p) chunk[n + threadIdx.x] = -1;
+        n += p;
     };
-    int first, end, step;
-    lensed_xcd_split(nitems, first, end, step);
-    for (int w = first; w < end; w += step) {
+    auto list_group = [&](int square0, bool lit) {
+        if (n + 64 + 3 > LENSED_CULL_CHUNK) {
+            pad();
+            flush();
+        }
+        const unsigned long long mask = __ballot(lit);
+        if (threadIdx.x < 64 && lit) chunk[n + __popcll(mask & ((1ull << lane) - 1ull))] = square0 + lane;
+        n += __popcll(mask);
+    };
+    int staged = -1;  // first item of the candidate whose squares the chunk ends with
+    int first, end;
+    lensed_xcd_run(nitems, first, end);
+    for (int w = first; w < end; ++w) {
         int s = lensed_find_sys<5>(sys, nsys, w);
         const lensed_desc& d = sys[s];
         const int groups = (d.ntile_r + LENSED_CULL_GROUP - 1) / LENSED_CULL_GROUP;
@@ -366,6 +361,10 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         const int g0 = (local - cand * groups) * LENSED_CULL_GROUP;
         const int ng = min(LENSED_CULL_GROUP, d.ntile_r - g0);
         const int item0 = d.item_r + cand * d.ntile_r + g0;
+        const int key = d.item_r + cand * d.ntile_r;  // block-uniform
+        const bool restage = key != staged;
+        if (restage) pad();  // entries never mix candidates
+        staged = key;
         if (!d.cull) {
             list_group(4 * item0, lane < 4 * ng);
             continue;
@@ -373,7 +372,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         switch (d.model) {
 #define LENSED_CULL_CASE(K)                                                                 \
     case K:                                                                                 \
-        lensed_cull_group<K>(d, cand, g0, ng, blocks, tables, npre, tab_lds, wave_lit);     \
+        lensed_cull_group<K>(d, cand, g0, ng, restage, blocks, tables, npre, tab_lds, wave_lit); \
         break;
             LENSED_FOR_MODELS(LENSED_CULL_CASE)
 #undef LENSED_CULL_CASE
@@ -405,6 +404,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
             }
         }
     }
+    pad();
     if (n > 0) flush();
 }
 
