@@ -266,23 +266,28 @@ __device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0,
     // the square's centre with rho grown by the square's half-diagonal holds
     // for every ray of the square, so a square found dark here needs no
     // per-pixel test (squares past the image edge are covered too)
-    bool dark_sq = false;
+    // A square whose own centre ray lands within the cutoff is kept without
+    // the per-pixel test (most such squares are lit, and keeping a square is
+    // always exact: the render then computes its values, 0 where they are 0)
+    bool dark_sq = false, lit_sq = false;
     {
         const int lane = (int)threadIdx.x & 63, t = lane >> 2, w = lane & 3;
         if (t < ng) {
             const int tile = g0 + t, by = tile / ntx, bx = tile - by * ntx;
-            const float px = (float)(bx * LENSED_RENDER_SIDE + (w & 1) * 8) + 3.5f;
-            const float py = (float)(by * LENSED_RENDER_SIDE + (w >> 1) * 8) + 3.5f;
+            const int x0 = bx * LENSED_RENDER_SIDE + (w & 1) * 8, y0 = by * LENSED_RENDER_SIDE + (w >> 1) * 8;
+            const float px = (float)x0 + 3.5f, py = (float)y0 + 3.5f;
             const float rho = 3.5f * sqrtf(sx * sx + sy * sy) + qrad;
             const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(ox + sx * px, oy + sy * py), true);
             dark_sq = lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, cc, rho);
+            lit_sq = !dark_sq && x0 < d.width && y0 < d.height &&
+                     !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, cc, 0.0f);
         }
     }
-    const unsigned long long dark = __ballot(dark_sq);
+    const unsigned long long dark = __ballot(dark_sq), kept = __ballot(lit_sq);
     const int wave = (int)threadIdx.x >> 6;
     for (int t = 0; t < ng; ++t) {
-        int any = 0;
-        if (!((dark >> (t * 4 + wave)) & 1ull)) {
+        int any = (kept >> (t * 4 + wave)) & 1ull ? 1 : 0;
+        if (!any && !((dark >> (t * 4 + wave)) & 1ull)) {
             int px, py;
             bool lit = false;
             if (lensed_render_pixel(d, g0 + t, wave, px, py)) {
