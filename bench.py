@@ -46,10 +46,10 @@ PEAK_HBM = 8000.0  # GB/s
 RAY_VALU = 35
 VALU_ISSUE_PER_S = 256 * 2.4e9  # wave instructions per second
 # HBM bytes per 4096-candidate render launch (prior draws, pixel cull on),
-# measured with rocprofv3 --pmc: FETCH_SIZE 146700 KiB (x 2, the factor a
-# 1 GiB copy calibrates) in and WRITE_SIZE 239650 KiB (x 1) out, the lit
+# measured with rocprofv3 --pmc: FETCH_SIZE 34190 KiB (x 2, the factor a
+# 1 GiB copy calibrates) in and WRITE_SIZE 236470 KiB (x 1) out, the lit
 # 8 x 8 squares of the model images (the cull pass writes the dark ones)
-TRAFFIC_RENDER_4096 = (146700 * 2 + 239650) * 1024
+TRAFFIC_RENDER_4096 = (34190 * 2 + 236470) * 1024
 
 
 def make_workload(K):
