@@ -4,7 +4,8 @@ Workload (the configuration the metric is quoted on): 256 x 256 px, EPL +
 external shear lens, Gaussian source (sigma 0.1" = 2 px), 21 x 21 Gaussian
 PSF (FWHM 3 px), g3k7 quadrature (49 rays per pixel), mock data at the true
 parameters plus seeded noise (sigma 0.05).  A step is one batched launch of
-4096 candidate parameter vectors drawn uniformly from the priors: parameters
+4096 candidate parameter vectors drawn uniformly from the priors, fresh
+draws every launch as a sampler's launches are: parameters
 in, one logL per candidate out (set_params -> render -> conv+chi^2 ->
 finalize, replayed from a captured hipGraph).
 
@@ -74,19 +75,28 @@ def make_workload(K):
 EMULATOR = os.path.join(ROOT, "lensed_amd", "data", "epl_mlp.bin")
 
 
+def draws(wl, ndim, K, n):
+    """n batches of K unit-cube draws, each launch its own as a sampler's
+    launches are (nothing a launch leaves behind matches the next one's
+    candidates); the last is the workload's seeded set."""
+    rng = np.random.default_rng(wl.cand_seed + 1)
+    return [rng.random((K, ndim)) for _ in range(n - 1)] + [wl.candidates(ndim, K)]
+
+
 def time_batched(wl, K, steps, warmup, flags):
-    """Seconds for `steps` launches of K candidates; logL of the last; the
-    emulator's statistics when the emulated path runs."""
+    """Seconds for `steps` launches of K candidates (fresh draws every
+    launch); logL of the last; the emulator's statistics when the emulated
+    path runs."""
     from lensed_amd._lib import LENSED_EMULATE
     from lensed_amd.lensed import Context
     emu = EMULATOR if flags & LENSED_EMULATE else None
     with Context(wl.spec(), max_batch=K, flags=flags, emulator=emu) as ctx:
-        cubes = wl.candidates(ctx.ndims(), K)
-        for _ in range(warmup):
-            ctx.loglike_batch(cubes)
+        sets = draws(wl, ctx.ndims(), K, warmup + steps)
+        for i in range(warmup):
+            ctx.loglike_batch(sets[i])
         t0 = time.perf_counter()
-        for _ in range(steps):
-            lnew, _ = ctx.loglike_batch(cubes)
+        for i in range(warmup, warmup + steps):
+            lnew, _ = ctx.loglike_batch(sets[i])
         dt = time.perf_counter() - t0
         stats = ctx.emulator_stats() if emu else None
     return dt, lnew, stats
@@ -118,19 +128,20 @@ def time_plain(wl, n):
 
 
 def kernel_times(wl, K, launches, flags=0):
-    """Stage times (ms per launch, hipEvents on the launch stream), with the
-    render stage split into the cull pass and the render kernel, and what
-    the render traced (the same candidates every launch)."""
+    """Stage times (ms per launch, hipEvents on the launch stream, fresh
+    draws every launch), with the render stage split into the cull pass and
+    the render kernel, and what the render traced in the last launch (the
+    workload's seeded candidates)."""
     from lensed_amd._lib import LENSED_EMULATE, LENSED_PROFILE
     from lensed_amd.lensed import Context
     emu = EMULATOR if flags & LENSED_EMULATE else None
     with Context(wl.spec(), max_batch=K, flags=LENSED_PROFILE | flags, emulator=emu) as ctx:
-        cubes = wl.candidates(ctx.ndims(), K)
-        ctx.loglike_batch(cubes)
+        sets = draws(wl, ctx.ndims(), K, launches + 1)
+        ctx.loglike_batch(sets[0])
         calls0, ms0 = ctx.profile()
         cull0 = ctx.render_stats()[3]
-        for _ in range(launches):
-            ctx.loglike_batch(cubes)
+        for i in range(launches):
+            ctx.loglike_batch(sets[1 + i])
         calls, ms = ctx.profile()
         items, kept, waves, cull1 = ctx.render_stats()
     n = calls - calls0

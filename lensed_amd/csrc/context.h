@@ -71,6 +71,7 @@ struct Layout {
     int grid_conv = 0, grid_render = 0;  // resident blocks for this launch's LDS use
     lensed_desc* d_desc = nullptr;
     hipGraphExec_t graph = nullptr;
+    int id = 0;                     // tells the launches of this layout apart on the device (owner)
 };
 
 }  // namespace lsd
@@ -120,6 +121,10 @@ struct lensed {
     size_t cap_npre = 0;
     int* d_trace = nullptr;      // per render item and wave: trace its rays (lensed_cull)
     int* d_live = nullptr;       // count, then the render items lensed_cull kept
+    int* d_owner = nullptr;      // [0] layout that last wrote the model images, [1] whether the
+                                 // current launch's layout is that one (set by lensed_set_params)
+    int* d_written = nullptr;    // per render square: its model values may be nonzero (lensed_cull)
+    int next_layout_id = 0;
     size_t cap_trace = 0;
     // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem
     hipEvent_t ev[10] = {};  // the eight stages' bounds, then the end of the cull pass

@@ -81,11 +81,18 @@ __device__ inline void lensed_node_pairs(const lensed_desc& d, u32* blk, const f
 extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const float* __restrict__ params, u32* __restrict__ blocks, const float4* __restrict__ quad,
-    float4* __restrict__ npre, int* __restrict__ live)
+    float4* __restrict__ npre, int* __restrict__ live, int* __restrict__ owner, int layout)
 {
     int item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= nitems) return;
-    if (item == 0 && live) live[0] = 0;  // the cull pass's list of lit render items starts empty
+    if (item == 0) {
+        if (live) live[0] = 0;  // the cull pass's list of lit squares starts empty
+        // owner[0]: the launch layout that last wrote the model images;
+        // owner[1]: whether that is this one, so that the cull pass may trust
+        // its record of which squares hold nonzero values
+        owner[1] = owner[0] == layout ? 1 : 0;
+        owner[0] = layout;
+    }
     int s = lensed_find_sys<2>(sys, nsys, item);
     const lensed_desc& d = sys[s];
     int cand = item - d.item_s;
@@ -314,11 +321,18 @@ static_assert(LENSED_RENDER_BS == 256, "a render item is four waves");
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const u32* __restrict__ blocks, const float4* __restrict__ tables, const float4* __restrict__ npre,
-    int* __restrict__ trace, int* __restrict__ live, float* __restrict__ model, float* __restrict__ errp)
+    int* __restrict__ trace, int* __restrict__ live, float* __restrict__ model, float* __restrict__ errp,
+    const int* __restrict__ owner, int* __restrict__ written)
 {
     extern __shared__ float4 tab_lds[];
     __shared__ int chunk[LENSED_CULL_CHUNK];
     __shared__ int wave_lit[LENSED_CULL_GROUP * (LENSED_RENDER_BS / 64)];
+    __shared__ int zero_sq[LENSED_CULL_GROUP * (LENSED_RENDER_BS / 64)];
+    // written[square]: the square's model values may be nonzero (a render
+    // wrote it).  Trusted when this layout wrote the model images last and no
+    // error plane is asked for (that plane is written only by such launches);
+    // otherwise every dark square is zeroed.
+    const bool trust = owner[1] != 0 && !errp;
     __shared__ int at;
     // the chunk's fill, the same in every thread (what is listed is
     // block-uniform)
@@ -383,20 +397,27 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
 #undef LENSED_CULL_CASE
             default: break;
         }
-        // the render's per-wave flags of the group's items, one coalesced store
-        if ((int)threadIdx.x < ng * (LENSED_RENDER_BS / 64))
-            trace[(long long)item0 * (LENSED_RENDER_BS / 64) + threadIdx.x] = wave_lit[threadIdx.x];
-        // list the lit squares; the dark ones get their zeros here: a dark
-        // item in rows of 16 pixels per 16 threads, a dark square of a lit
-        // item by its own wave
+        // the render's per-wave flags of the group's items, one coalesced
+        // store; which dark squares need their zeros written
+        if ((int)threadIdx.x < ng * (LENSED_RENDER_BS / 64)) {
+            const long long sqi = (long long)item0 * (LENSED_RENDER_BS / 64) + threadIdx.x;
+            const int lit = wave_lit[threadIdx.x];
+            trace[sqi] = lit;
+            zero_sq[threadIdx.x] = !lit && (!trust || written[sqi]) ? 1 : 0;
+            written[sqi] = lit;
+        }
+        __syncthreads();  // zero_sq complete
+        // list the lit squares; the dark ones that may hold nonzero values
+        // get their zeros here: a wholly such item in rows of 16 pixels per
+        // 16 threads, otherwise each square by its own wave
         const int ntx = (d.width + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
         const int wave = (int)threadIdx.x >> 6;
         list_group(4 * item0, lane < 4 * ng && wave_lit[lane] != 0);
         for (int t = 0; t < ng; ++t) {
-            const int* f = wave_lit + t * (LENSED_RENDER_BS / 64);
+            const int* z = zero_sq + t * (LENSED_RENDER_BS / 64);
             int px = -1, py = -1;
-            if (f[0] | f[1] | f[2] | f[3]) {
-                if (!f[wave] && !lensed_render_pixel(d, g0 + t, wave, px, py)) px = -1;
+            if (!(z[0] & z[1] & z[2] & z[3])) {
+                if (z[wave] && !lensed_render_pixel(d, g0 + t, wave, px, py)) px = -1;
             } else {
                 const int tile = g0 + t, by = tile / ntx, bx = tile - by * ntx;
                 px = bx * LENSED_RENDER_SIDE + ((int)threadIdx.x & 15);

@@ -349,14 +349,19 @@ def test_c3_culling_is_exact(c3):
     edge[:, pids.index("lens.x")] = 128.0
     edge[:, pids.index("lens.y")] = 128.0
     edge[:, pids.index("lens.t")] = [0.6, 0.999, 1.0, 1.4]
+    # a second batch of other candidates in the same launch layout: the cull
+    # then writes zeros only where the first batch left nonzero values
+    cubes2 = np.random.default_rng(5).random((1024, 14))
     res, render_ms, stats = [], [], []
     for extra in (0, LENSED_NO_CULL):
         with Context(wl.spec(), max_batch=1024, flags=LENSED_TABLE | LENSED_GRAPH | extra) as ctx:
             lc, _ = ctx.loglike_batch(cubes)
             stats.append(ctx.render_stats()[:3])
+            lc2, _ = ctx.loglike_batch(cubes2)
             ln = ctx.loglike_phys(np.concatenate([phys, edge]))
             planes = [ctx.render(p) for p in (edge[0], edge[2], edge[3], phys[0])]
-            res.append((lc, ln, planes))
+            lc3, _ = ctx.loglike_batch(cubes)
+            res.append((np.concatenate([lc, lc2, lc3]), ln, planes))
         with Context(wl.spec(), max_batch=1024, flags=LENSED_TABLE | LENSED_PROFILE | extra) as ctx:
             ctx.loglike_batch(cubes)
             calls, ms = ctx.profile()
