@@ -303,8 +303,8 @@ def main():
                     help="only run device-to-device copies of MIB MiB (byte-counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: candidates are independent, so the path shards by splitting a "
-              "launch's candidates across GPUs (see DESIGN.md); multi-GPU runs are not part of this build.",
+        print("bench.py runs on one GPU: candidates are independent, so the path would shard by splitting a "
+              "launch's candidates across GPUs; runs on several GPUs are not implemented.",
               file=sys.stderr)
         sys.exit(2)
 

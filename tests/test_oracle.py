@@ -50,7 +50,7 @@ def test_oracle_reproduces_reference_renders(golden, dtype):
         rel = np.max(np.abs(value - img)[mask] / np.abs(img[mask]))
         worst[t] = rel
         # the reference renders are FP32 with fast-relaxed OpenCL maths;
-        # atan/atanh of the SIE family differ most (SURVEY probe: 3.0e-5)
+        # atan/atanh of the SIE family differ most (up to 3.0e-5 of the peak)
         assert np.max(np.abs(value - img)) <= 3e-5 * peak, t
         assert rel <= 8e-5, (t, rel)
         # ERR layer = quadrature error / value
