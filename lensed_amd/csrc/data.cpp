@@ -3,7 +3,7 @@
 //
 // Lensed reads its inputs through CFITSIO (src/data.c:54-118) and writes the
 // dumper's layers as a multi-extension FITS file (src/data.c:120-165,
-// 372-391).  CFITSIO is not part of this build; this file reads and writes
+// 372-391).  CFITSIO is not linked; this file reads and writes
 // the part of the FITS standard those files use: a primary HDU and IMAGE
 // extensions, BITPIX 8/16/32/64/-32/-64 with BSCALE/BZERO, big-endian data
 // in 2880-byte records, and CFITSIO's file-name syntax for an extension

@@ -3,7 +3,7 @@
 // Lensed hands its likelihood to MultiNest's run() (src/lensed.c:1236-1288),
 // which calls loglike() once per candidate and dumper() every updInt
 // iterations (src/nested.c:17,133).  MultiNest is an external Fortran library
-// that is not part of this build; this file provides a nested sampler with
+// that is not linked here; this file provides a nested sampler with
 // the same entry point and callbacks, and a batched variant that proposes K
 // points at a time so one launch evaluates them all.
 //
