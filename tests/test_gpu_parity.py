@@ -418,3 +418,45 @@ def test_culling_ragged_image_is_exact(shape, psf_size):
     items = 256 * ((h + 15) // 16) * ((w + 15) // 16)
     assert stats[1] == (items, items, 4 * items)
     assert stats[0][0] == items and 0 < stats[0][1] < items, stats
+
+
+def test_culling_two_systems_in_one_launch():
+    """Two culled systems of different sizes (256 x 256 and 70 x 33) in one
+    launch, two batches of other candidates in a row (the second trusts the
+    cull's record of which squares the first wrote): logL bit for bit that
+    of tracing every ray, for both systems and both batches."""
+    from lensed_amd._lib import LENSED_NO_CULL, LENSED_TABLE
+    big = W.c3()
+    big.make_data(oracle_render_cb, [f"{oid}.{p}" for oid, n in big.objects for p in O.PARAMS[n]],
+                  mag_keys=("source.mag",))
+    h, w = 70, 33
+    cx, cy = (w + 1) / 2, (h + 1) / 2
+    objs = [("lens", "epl_plus_shear"), ("source", "gauss")]
+    pri = {"lens.x": f"unif {cx - 1} {cx + 1}", "lens.y": f"unif {cy - 1} {cy + 1}", "lens.r": "unif 4 8",
+           "lens.t": "unif 0.5 1.5", "lens.q": "unif 0.3 1", "lens.pa": "unif 0 180", "lens.g1": "unif -0.1 0.1",
+           "lens.g2": "unif -0.1 0.1", "source.x": f"unif {cx - 3} {cx + 3}", "source.y": f"unif {cy - 3} {cy + 3}",
+           "source.sigma": "unif 0.3 0.8", "source.mag": "unif -5 5", "source.q": "unif 0.4 1",
+           "source.pa": "unif 0 180"}
+    rng = np.random.default_rng(17)
+    small = SystemSpec(objs, pri, rng.normal(0, 0.05, (h, w)).astype(np.float32),
+                       np.full((h, w), 400.0, np.float32), psf=W.gaussian_psf(5, 2.0))
+    specs = [big.spec(), small]
+    out = []
+    for extra in (0, LENSED_NO_CULL):
+        with Context(specs, max_batch=256, flags=LENSED_TABLE | LENSED_GRAPH | extra) as ctx:
+            batches = []
+            for seed in (1, 2):
+                r = np.random.default_rng(seed)
+                phys = []
+                for sidx in range(2):
+                    cubes = r.random((256 if sidx == 0 else 192, ctx.ndims(sidx)))
+                    _, p = ctx.loglike_batch(cubes, sidx)
+                    full = np.zeros((len(p), ctx.npars(sidx)))
+                    full[:, ctx.pmap(sidx)] = p
+                    phys.append(full)
+                batches.append(phys)
+            # the two-system launches back to back, in the same layout
+            out.append([ctx.loglike_phys(phys) for phys in batches])
+    for (a0, a1), (b0, b1) in zip(zip(*out[0]), zip(*out[1])):
+        assert np.all(np.isfinite(a0)) and np.array_equal(a0, b0)
+        assert np.array_equal(a1, b1)
