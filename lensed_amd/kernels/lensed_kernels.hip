@@ -560,7 +560,12 @@ __device__ inline void lensed_conv_generic(const lensed_desc& d, const float* __
     const int PW = d.psfw, PH = d.psfh;
     const int stride = LENSED_TILE_STRIDE(PW), ch = TH + PH - 1;
     if (d.sep_rank > 0) {
-        float* tmp = tile + stride * ch;  // row-pass results, ch x TX
+        // row-pass results, ch rows: over the tile itself for rank 1 (a row
+        // is read and then written by one wave, whose LDS accesses complete
+        // in order), after it for higher ranks, which read the tile again
+        const bool inplace = d.sep_rank == 1;
+        float* tmp = inplace ? tile : tile + stride * ch;
+        const int ts = inplace ? stride : TX;
         for (int k = 0; k < d.sep_rank; ++k) {
             const float* cf = psf + d.sep_off + (long long)k * (PH + PW);
             const float* rf = cf + PH;
@@ -569,14 +574,14 @@ __device__ inline void lensed_conv_generic(const lensed_desc& d, const float* __
                 const float* src = tile + r * stride + x + PW - 1;
                 float sum = 0.0f;
                 for (int i = 0; i < PW; ++i) sum += rf[i] * src[-i];
-                tmp[idx] = sum;
+                tmp[r * ts + x] = sum;
             }
             __syncthreads();
             for (int j = 0; j < PH; ++j) {
                 const float w = cf[j];
-                const float* col = tmp + (ly * RY + PH - 1 - j) * TX + lx;
+                const float* col = tmp + (ly * RY + PH - 1 - j) * ts + lx;
 #pragma unroll
-                for (int r = 0; r < RY; ++r) acc[r] += w * col[r * TX];
+                for (int r = 0; r < RY; ++r) acc[r] += w * col[r * ts];
             }
             __syncthreads();
         }
@@ -600,45 +605,61 @@ __device__ inline void lensed_conv_generic(const lensed_desc& d, const float* __
 // the column pass RY outputs from RY + PH - 1 values.  Direct: per PSF
 // column, RY + PH - 1 values feed RY x PH multiply-adds (taps stored
 // transposed so a column's taps are contiguous).
+// One rank of the separable convolution: row pass into `tmp` (row stride
+// TS), then the column pass into acc.  With TMP_IN_TILE (rank 1) the row
+// pass writes over the tile itself: a row is read and then written by one
+// wave, whose LDS accesses complete in order.
+template <int RY, int PW, int PH, bool TMP_IN_TILE>
+__device__ inline void lensed_conv_sep_rank(const float* __restrict__ cf, float* tile, int lx, int ly,
+                                            float (&acc)[RY])
+{
+    constexpr int TX = LENSED_CONV_TX, TH = LENSED_CONV_TY * RY, BS = LENSED_CONV_BS;
+    constexpr int CH = TH + PH - 1, STRIDE = LENSED_TILE_STRIDE(PW), NC = RY + PH - 1;
+    constexpr int NX = 4, NV = (NX + PW - 1 + 3) / 4;
+    constexpr int TS = TMP_IN_TILE ? STRIDE : TX;
+    float* tmp = TMP_IN_TILE ? tile : tile + STRIDE * CH;
+    const float* rf = cf + PH;
+    for (int r = threadIdx.x / (TX / NX); r < CH; r += BS / (TX / NX)) {
+        const int x0 = (threadIdx.x % (TX / NX)) * NX;
+        const float4* src = (const float4*)(tile + r * STRIDE + x0);
+        float v[4 * NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            float4 f = src[q];
+            v[4 * q] = f.x; v[4 * q + 1] = f.y; v[4 * q + 2] = f.z; v[4 * q + 3] = f.w;
+        }
+        float o[NX];
+#pragma unroll
+        for (int x = 0; x < NX; ++x) o[x] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < PW; ++i)
+#pragma unroll
+            for (int x = 0; x < NX; ++x) o[x] += rf[i] * v[x + PW - 1 - i];
+        *(float4*)(tmp + r * TS + x0) = make_float4(o[0], o[1], o[2], o[3]);
+    }
+    __syncthreads();
+    float col[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) col[q] = tmp[(ly * RY + q) * TS + lx];
+#pragma unroll
+    for (int j = 0; j < PH; ++j)
+#pragma unroll
+        for (int r = 0; r < RY; ++r) acc[r] += cf[j] * col[r + PH - 1 - j];
+    __syncthreads();
+}
+
 template <int RY, int PW, int PH, bool SEP>
 __device__ inline void lensed_conv_fixed(const lensed_desc& d, const float* __restrict__ psf, float* tile,
                                          int lx, int ly, float (&acc)[RY])
 {
-    constexpr int TX = LENSED_CONV_TX, TH = LENSED_CONV_TY * RY, BS = LENSED_CONV_BS;
-    constexpr int CH = TH + PH - 1, STRIDE = LENSED_TILE_STRIDE(PW), NC = RY + PH - 1;
+    constexpr int STRIDE = LENSED_TILE_STRIDE(PW), NC = RY + PH - 1;
     if (SEP) {
-        constexpr int NX = 4, NV = (NX + PW - 1 + 3) / 4;
-        float* tmp = tile + STRIDE * CH;
-        for (int k = 0; k < d.sep_rank; ++k) {
-            const float* cf = psf + d.sep_off + (long long)k * (PH + PW);
-            const float* rf = cf + PH;
-            for (int r = threadIdx.x / (TX / NX); r < CH; r += BS / (TX / NX)) {
-                const int x0 = (threadIdx.x % (TX / NX)) * NX;
-                const float4* src = (const float4*)(tile + r * STRIDE + x0);
-                float v[4 * NV];
-#pragma unroll
-                for (int q = 0; q < NV; ++q) {
-                    float4 f = src[q];
-                    v[4 * q] = f.x; v[4 * q + 1] = f.y; v[4 * q + 2] = f.z; v[4 * q + 3] = f.w;
-                }
-                float o[NX];
-#pragma unroll
-                for (int x = 0; x < NX; ++x) o[x] = 0.0f;
-#pragma unroll
-                for (int i = 0; i < PW; ++i)
-#pragma unroll
-                    for (int x = 0; x < NX; ++x) o[x] += rf[i] * v[x + PW - 1 - i];
-                *(float4*)(tmp + r * TX + x0) = make_float4(o[0], o[1], o[2], o[3]);
-            }
-            __syncthreads();
-            float col[NC];
-#pragma unroll
-            for (int q = 0; q < NC; ++q) col[q] = tmp[(ly * RY + q) * TX + lx];
-#pragma unroll
-            for (int j = 0; j < PH; ++j)
-#pragma unroll
-                for (int r = 0; r < RY; ++r) acc[r] += cf[j] * col[r + PH - 1 - j];
-            __syncthreads();
+        if (d.sep_rank == 1) {
+            lensed_conv_sep_rank<RY, PW, PH, true>(psf + d.sep_off, tile, lx, ly, acc);
+        } else {
+            for (int k = 0; k < d.sep_rank; ++k)
+                lensed_conv_sep_rank<RY, PW, PH, false>(psf + d.sep_off + (long long)k * (PH + PW), tile, lx, ly,
+                                                        acc);
         }
     } else {
         const float* pt = psf + d.psft_off;  // [PW][PH]
@@ -803,7 +824,9 @@ __device__ inline void lensed_conv_chi2_body(const lensed_desc* __restrict__ sys
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) run(item);
 }
 
-extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) void lensed_conv_chi2(
+// six waves per SIMD (at most 80 VGPRs): with the rank-1 row pass over the
+// tile (18 KiB of LDS for a 21 x 21 PSF) six blocks fit a CU
+extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) __attribute__((amdgpu_waves_per_eu(6))) void lensed_conv_chi2(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const float* __restrict__ model, const float* __restrict__ image,
     const float* __restrict__ weight, const float* __restrict__ psf,
