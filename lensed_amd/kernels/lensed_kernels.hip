@@ -238,17 +238,18 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 }
 
 // -------------------------------------------------------------------- cull
-// Before the render: for each render item, whether any of its pixels can
-// receive light (lensed_frame_cull in epl_table.h bounds where a pixel's
-// rays land).  Items with no lit pixel get their zero model values here and
-// are left out of the render; the others go to a list that the render's
-// blocks share out evenly (lit items bunch in the middle of the image, so a
-// fixed item-to-block mapping would leave the blocks of the edge tiles idle),
-// with one flag per wave for the waves of a listed item that stay dark.
-// A cull work item is LENSED_CULL_GROUP render items of one candidate: the
-// table is staged once for all of them and the waves meet at one barrier.
+// Before the render: for each 8 x 8 square of each render item (a wave's
+// pixels), whether any of its pixels can receive light (lensed_frame_cull in
+// epl_table.h bounds where a pixel's rays land).  Dark squares get their zero
+// model values here (where they may be missing) and are left out of the
+// render; the lit ones go to a list that the render's blocks share out
+// evenly (lit squares bunch in the middle of the image, so a fixed
+// item-to-block mapping would leave the blocks of the edge tiles idle).  One
+// flag per square (`trace`) tells the convolution which tiles are all dark.
+// A cull work item is LENSED_CULL_GROUP render items of one candidate; a
+// block takes consecutive ones and stages a candidate's table once for all.
 // Only systems with d.cull (a tabulated EPL lens and one framed source with a
-// cutoff) are tested; the items of the others are all listed.
+// cutoff) are tested; the squares of the others are all listed.
 template <int K>
 __device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0, int ng, bool restage,
                                          const u32* __restrict__ blocks, const float4* __restrict__ tables,
@@ -397,7 +398,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
 #undef LENSED_CULL_CASE
             default: break;
         }
-        // the render's per-wave flags of the group's items, one coalesced
+        // the group's per-square flags (for the convolution), one coalesced
         // store; which dark squares need their zeros written
         if ((int)threadIdx.x < ng * (LENSED_RENDER_BS / 64)) {
             const long long sqi = (long long)item0 * (LENSED_RENDER_BS / 64) + threadIdx.x;
