@@ -253,7 +253,8 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 template <int K>
 __device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0, int ng, bool restage,
                                          const u32* __restrict__ blocks, const float4* __restrict__ tables,
-                                         const float4* __restrict__ npre, float4* tab_lds, int* wave_lit)
+                                         const float4* __restrict__ npre, float4* tab_lds, int* wave_lit,
+                                         unsigned long long* sq_masks)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
@@ -277,8 +278,10 @@ __device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0,
     // A square whose own centre ray lands within the cutoff is kept without
     // the per-pixel test (most such squares are lit, and keeping a square is
     // always exact: the render then computes its values, 0 where they are 0)
-    bool dark_sq = false, lit_sq = false;
-    {
+    // (computed by the first wave for the block's 64 squares, shared in LDS)
+    const int wave = (int)threadIdx.x >> 6;
+    if (wave == 0) {
+        bool dark_sq = false, lit_sq = false;
         const int lane = (int)threadIdx.x & 63, t = lane >> 2, w = lane & 3;
         if (t < ng) {
             const int tile = g0 + t, by = tile / ntx, bx = tile - by * ntx;
@@ -290,9 +293,14 @@ __device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0,
             lit_sq = !dark_sq && x0 < d.width && y0 < d.height &&
                      !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, cc, 0.0f);
         }
+        const unsigned long long dk = __ballot(dark_sq), kp = __ballot(lit_sq);
+        if (lane == 0) {
+            sq_masks[0] = dk;
+            sq_masks[1] = kp;
+        }
     }
-    const unsigned long long dark = __ballot(dark_sq), kept = __ballot(lit_sq);
-    const int wave = (int)threadIdx.x >> 6;
+    __syncthreads();  // the masks are published (read before the group's last barrier)
+    const unsigned long long dark = sq_masks[0], kept = sq_masks[1];
     for (int t = 0; t < ng; ++t) {
         int any = (kept >> (t * 4 + wave)) & 1ull ? 1 : 0;
         if (!any && !((dark >> (t * 4 + wave)) & 1ull)) {
@@ -329,6 +337,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     __shared__ int chunk[LENSED_CULL_CHUNK];
     __shared__ int wave_lit[LENSED_CULL_GROUP * (LENSED_RENDER_BS / 64)];
     __shared__ int zero_sq[LENSED_CULL_GROUP * (LENSED_RENDER_BS / 64)];
+    __shared__ unsigned long long sq_masks[2];
     // written[square]: the square's model values may be nonzero (a render
     // wrote it).  Trusted when this layout wrote the model images last and no
     // error plane is asked for (that plane is written only by such launches);
@@ -392,7 +401,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         switch (d.model) {
 #define LENSED_CULL_CASE(K)                                                                 \
     case K:                                                                                 \
-        lensed_cull_group<K>(d, cand, g0, ng, restage, blocks, tables, npre, tab_lds, wave_lit); \
+        lensed_cull_group<K>(d, cand, g0, ng, restage, blocks, tables, npre, tab_lds, wave_lit, sq_masks); \
         break;
             LENSED_FOR_MODELS(LENSED_CULL_CASE)
 #undef LENSED_CULL_CASE
