@@ -58,6 +58,10 @@ int orc_type(int kind) { return kind >= 0 && kind < ORC_NKINDS ? ORC_TYPES[kind]
 #undef REAL
 #undef SFX
 
+/* Sersic b(n) of objects/sersic.cl:32-33 in FP64, for the check against its
+ * defining equation Gamma(2n, b) / Gamma(2n) = 1/2 (docs/sources.md:109-111) */
+double orc_sersic_b(double n) { return sersic_b_d(n); }
+
 /* chi^2 = sum w (m - d)^2 with every term in FP32 and the sum in FP64,
  * kernel/lensed.cl:50-51 + src/nested.c:107-109 */
 double orc_chi2_f32terms(int n, const float* image, const float* weight, const float* model) {

@@ -62,6 +62,8 @@ def _load(name):
         lib.orc_chi2_f32terms.restype = c_double
         lib.orc_chi2_d.argtypes = [c_int, dp, dp, dp]
         lib.orc_chi2_d.restype = c_double
+        lib.orc_sersic_b.argtypes = [c_double]
+        lib.orc_sersic_b.restype = c_double
         lib.orc_gauss_quantile.argtypes = [c_double]
         lib.orc_gauss_quantile.restype = c_double
         _libs[name] = lib
@@ -160,6 +162,11 @@ def chi2_f32terms(image, weight, model):
 def chi2_d(image, weight, model):
     a = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (image, weight, model)]
     return lib().orc_chi2_d(len(a[0]), *[_ptr(v, c_double) for v in a])
+
+
+def sersic_b(n):
+    """b(n) of the Sersic profile (objects/sersic.cl:32-33), FP64."""
+    return lib().orc_sersic_b(float(n))
 
 
 def gauss_quantile(u):

@@ -11,6 +11,9 @@ Which check pins which oracle function:
                                SIE convergence; rc = 0 NSIE equals SIE
   SIS / NSIS / point mass      closed forms
   source normalisations        total flux = 10^(-0.4 mag) for circular profiles
+  Sersic b(n)                  the root of its defining equation
+                               Gamma(2n, b)/Gamma(2n) = 1/2 (scipy), within the
+                               series' own error; n = 1/2 Sersic is a Gaussian
   set_params image-plane prior traced position equals x - alpha(x)
   convolve                     scipy convolution of an edge-padded image,
                                delta PSF = identity, odd and even PSF sizes
@@ -20,6 +23,7 @@ Which check pins which oracle function:
 import numpy as np
 import pytest
 import scipy.signal
+import scipy.special
 import scipy.stats
 
 from conftest import fixture_objects
@@ -163,13 +167,62 @@ def test_source_total_flux(name, size):
     assert value.sum() == pytest.approx(10 ** (-0.4 * mag), rel=tol)
 
 
+def sersic_b_exact(n):
+    """The root b of Gamma(2n, b) / Gamma(2n) = 1/2, the definition of b(n)
+    (half the light within the effective radius, docs/sources.md:109-111)."""
+    return scipy.special.gammainccinv(2 * n, 0.5)
+
+
+def test_sersic_b_solves_its_defining_equation():
+    # objects/sersic.cl:32-33 uses the asymptotic series of Ciotti & Bertin
+    # (1999) for n > 0.36.  Its error against the exact root falls like n^-3
+    # (its n^-3 coefficient 131/1148175 differs from the exact expansion's,
+    # Choi's 281/3444525, by 3.3e-5), so the bound is 4.5e-5 / n^3 from n = 1
+    # on and 3e-4 below; at n = 8..100 that is tight enough that changing any
+    # coefficient's sign, or the leading ones by 1e-3 of their value, fails.
+    for n in np.concatenate([np.linspace(0.37, 1.0, 30), np.linspace(1.0, 8.0, 50), [12.0, 20.0, 50.0, 100.0]]):
+        err = O.sersic_b(n) - sersic_b_exact(n)
+        bound = 4.5e-5 / n ** 3 if n >= 1 else 3e-4
+        assert abs(err) <= bound, (n, err, bound)
+    # n <= 0.36: the polynomial fit of MacArthur, Courteau & Holtzman (2003),
+    # good to 1.2 % of b down to n = 0.2
+    for n in np.linspace(0.2, 0.36, 17):
+        err = O.sersic_b(n) - sersic_b_exact(n)
+        assert abs(err) <= 0.012 * sersic_b_exact(n) and abs(err) <= 2e-3, (n, err)
+    # the two branches meet within their errors
+    assert abs(O.sersic_b(0.36) - O.sersic_b(0.3600001)) < 5e-4
+
+
+def test_sersic_b_bound_catches_a_planted_error():
+    # the check above, run on b(n) with one coefficient of the series
+    # altered, fails: it pins the coefficients, not only the leading 2n - 1/3
+    def series(n, c):
+        return 2 * n - 1 / 3 + c[0] / n + c[1] / n ** 2 + c[2] / n ** 3 + c[3] / n ** 4
+
+    good = [4 / 405, 46 / 25515, 131 / 1148175, -2194697 / 30690717750]
+    ns = np.concatenate([np.linspace(1.0, 8.0, 50), [12.0, 20.0, 50.0, 100.0]])
+
+    def passes(c):
+        return all(abs(series(n, c) - sersic_b_exact(n)) <= 4.5e-5 / n ** 3 for n in ns)
+
+    assert passes(good)
+    assert abs(series(2.5, good) - O.sersic_b(2.5)) < 1e-13
+    for k in range(4):
+        bad = list(good)
+        bad[k] = -bad[k]
+        assert not passes(bad), k
+    for k in range(2):
+        bad = list(good)
+        bad[k] *= 1 + 1e-3
+        assert not passes(bad), k
+
+
 def test_sersic_half_is_gaussian():
-    # Sersic n = 1/2 is a Gaussian with sigma^2 = r^2/(2 b)
+    # Sersic n = 1/2 is a Gaussian with sigma^2 = r^2/(2 b), b = b(1/2)
     objs = [("s", "sersic")]
     n, r = 0.5, 4.0
     v, _ = O.render(objs, [40.5, 40.5, r, 0.0, n, 1.0, 0.0], (80, 80), QQ, WW)
-    b = 2 * n - 1 / 3 + 4 / (405 * n) + 46 / (25515 * n ** 2) + 131 / (1148175 * n ** 3) - 2194697 / (30690717750 * n ** 4)
-    sig = r / np.sqrt(2 * b)
+    sig = r / np.sqrt(2 * O.sersic_b(n))
     g, _ = O.render([("g", "gauss")], [40.5, 40.5, sig, 0.0, 1.0, 0.0], (80, 80), QQ, WW)
     assert np.allclose(v, g, rtol=1e-9, atol=1e-14)
 
