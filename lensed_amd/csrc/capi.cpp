@@ -160,7 +160,7 @@ char* lensed_model_source(size_t nobjs, const object* objs) {
 char* lensed_model_source_tabulated(size_t nobjs, const object* objs) {
     std::vector<ModelObject> model = model_of(nobjs, objs);
     for (auto& mo : model)
-        if (mo.name == "epl" || mo.name == "epl_plus_shear") {
+        if (mo.type == OBJ_LENS && object_tabulates(mo.name)) {
             mo.table = true;
             break;
         }

@@ -72,7 +72,12 @@ struct Layout {
     lensed_desc* d_desc = nullptr;
     hipGraphExec_t graph = nullptr;
     int id = 0;                     // tells the launches of this layout apart on the device (owner)
+    unsigned long long last_use = 0;  // context's launch clock at its last use (eviction order)
 };
+
+// cached launch layouts per context; beyond this many the least recently
+// used one is freed (its descriptors and graph)
+constexpr size_t kMaxLayouts = 32;
 
 }  // namespace lsd
 
@@ -125,6 +130,7 @@ struct lensed {
                                  // current launch's layout is that one (set by lensed_set_params)
     int* d_written = nullptr;    // per render square: its model values may be nonzero (lensed_cull)
     int next_layout_id = 0;
+    unsigned long long layout_clock = 0;
     size_t cap_trace = 0;
     // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem
     hipEvent_t ev[10] = {};  // the eight stages' bounds, then the end of the cull pass

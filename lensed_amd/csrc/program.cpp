@@ -244,20 +244,43 @@ static std::string data_ptr(const std::string& m, size_t off, const char* base) 
     return o.str();
 }
 
+// the plugin's source text, empty if there is none (read on every call: the
+// object search path may change between contexts)
+static std::string plugin_text(const std::string& name) {
+    std::string src;
+    if (!load_object_source(name, &src, nullptr)) src.clear();
+    return src;
+}
+
 // whether a plugin defines the functions named by the patterns
 static bool plugin_defines(const std::string& name, const std::vector<const char*>& patterns) {
-    static std::mutex mu;
-    static std::map<std::string, std::string> sources;
-    std::lock_guard<std::mutex> lock(mu);
-    auto it = sources.find(name);
-    if (it == sources.end()) {
-        std::string src;
-        if (!load_object_source(name, &src, nullptr)) src.clear();
-        it = sources.emplace(name, src).first;
-    }
-    bool ok = !it->second.empty();
-    for (const char* re : patterns) ok = ok && std::regex_search(it->second, std::regex(re));
+    const std::string src = plugin_text(name);
+    bool ok = !src.empty();
+    for (const char* re : patterns) ok = ok && std::regex_search(src, std::regex(re));
     return ok;
+}
+
+// whether the plugin's data block declares every field of the patterns
+static bool data_declares(const std::string& name, const std::vector<const char*>& fields) {
+    // comments removed first: they may hold braces
+    const std::string src =
+        std::regex_replace(std::regex_replace(plugin_text(name), std::regex(R"(/\*[\s\S]*?\*/)"), " "),
+                           std::regex(R"(//[^\n]*)"), " ");
+    std::smatch m;
+    if (!std::regex_search(src, m, std::regex(R"(\bdata\s*\{([^}]*)\})"))) return false;
+    const std::string block = m[1].str();
+    for (const char* re : fields)
+        if (!std::regex_search(block, std::regex(re))) return false;
+    return true;
+}
+
+bool object_tabulates(const std::string& name) {
+    return data_declares(name, {R"(\bfloat2\s+c\s*;)", R"(\bmat22\s+fwd\s*;)", R"(\bmat22\s+bwd\s*;)",
+                                R"(\bfloat\s+coef\s*\[\s*10\s*\]\s*;)", R"(\bfloat\s+hexp\s*;)"});
+}
+
+bool object_table_shear(const std::string& name) {
+    return object_tabulates(name) && data_declares(name, {R"(\bmat22\s+shear\s*;)"});
 }
 
 static bool source_has_frame(const std::string& name) {
@@ -303,7 +326,7 @@ std::string compute_function(const std::vector<ModelObject>& objs, int k) {
             if (objs[i].table && lm.empty()) {
                 lens_off = dd;
                 lm = mangle(objs[i].name);
-                if (objs[i].name == "epl_plus_shear") pre_fn += "_shear";
+                if (object_table_shear(objs[i].name)) pre_fn += "_shear";
             }
             if ((int)i == fsrc) src_off = dd, sm = mangle(objs[i].name);
             dd += objs[i].size;

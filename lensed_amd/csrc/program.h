@@ -50,6 +50,15 @@ struct ModelObject {
     bool table = false;     // EPL deflection with the tabulated angular factor
 };
 
+// Whether a lens plugin can take the tabulated EPL angular factor: the table
+// form reads the lens's data block by field name, so its data block must
+// declare float2 c (centre), mat22 fwd (image -> elliptical frame, scaled),
+// mat22 bwd (back, with the normalisation), float coef[10] (the series'
+// coefficients) and float hexp ((1 - t)/2); the caller also needs its
+// parameters t and q.  With a mat22 shear field the shear is folded in too.
+bool object_tabulates(const std::string& name);
+bool object_table_shear(const std::string& name);
+
 // Generated code for one model (ordered object list): compute_<k>() and
 // set_params_<k>().  Offsets are in 4-byte words.
 std::string compute_function(const std::vector<ModelObject>& objs, int k);

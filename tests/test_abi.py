@@ -161,3 +161,49 @@ def test_reference_plugins_load_unmodified(monkeypatch):
         assert np.max(np.abs(own - ref)) <= 3e-5 * peak, objs
     src = program_for([("lens", "epl_plus_shear"), ("src", "sersic-old"), ("fg", "sky")])
     assert compile_program(src)[0] > 0
+
+
+def _tabulated_source(objects):
+    lib = L.load()
+    arr = (L.Object * len(objects))()
+    for i, (oid, name) in enumerate(objects):
+        assert lib.lensed_object_load(byref(arr[i]), oid.encode(), name.encode()) == 0, L.last_error()
+    return L.take_string(lib.lensed_model_source_tabulated(len(objects), arr)), \
+        L.take_string(lib.lensed_model_source(len(objects), arr))
+
+
+def test_table_form_follows_the_data_layout(monkeypatch, tmp_path):
+    """The tabulated EPL form reads the lens's data block by field name, so it
+    is chosen by what the block declares, not by the plugin's name: this
+    package's EPL objects get it; the same plugin with a renamed field, or
+    under another name, gets (or keeps) the per-ray series."""
+    objs = [("lens", "epl_plus_shear"), ("src", "gauss")]
+    tab, plain = _tabulated_source(objs)
+    assert "lensed_epl_table_deflection_pre" in tab or "lensed_epl_table_vector_pre" in tab
+    assert "lensed_epl_pre_shear" in tab
+    own = open(os.path.join(ROOT, "lensed_amd", "objects", "epl_plus_shear.cl")).read()
+    # another name, same layout: tabulated
+    (tmp_path / "my_power_law.cl").write_text(own)
+    # the centre field renamed: the table form cannot read it
+    (tmp_path / "epl_renamed.cl").write_text(own.replace("float2 c;", "float2 centre;")
+                                             .replace("this->c ", "this->centre ").replace("this->c;", "this->centre;")
+                                             .replace("this->c =", "this->centre ="))
+    monkeypatch.setenv("LENSED_OBJECTS_PATH", str(tmp_path))
+    tab2, _ = _tabulated_source([("lens", "my_power_law"), ("src", "gauss")])
+    assert "lensed_epl_table_vector_pre" in tab2 and "lensed_epl_pre_shear" in tab2
+    tab3, plain3 = _tabulated_source([("lens", "epl_renamed"), ("src", "gauss")])
+    assert tab3 == plain3 and "lensed_epl_table" not in tab3.split("#line 1 \"generated\"")[1]
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "objects")), reason="reference not present")
+def test_reference_epl_plugin_keeps_the_series_under_table_mode(monkeypatch):
+    """The reference's unmodified objects/epl_plus_shear.cl (fields x, m, w,
+    g, t, f, n) asks for the table form under LENSED_TABLE: it gets the
+    per-ray series, and the program builds for gfx950."""
+    monkeypatch.setenv("LENSED_OBJECTS_PATH", os.path.join(REFERENCE, "objects"))
+    objs = [("lens", "epl_plus_shear"), ("src", "gauss")]
+    tab, plain = _tabulated_source(objs)
+    assert tab == plain
+    assert "lensed_epl_table" not in tab.split("#line 1 \"generated\"")[1]
+    size, log = compile_program(program_for(objs))
+    assert size > 0, log
