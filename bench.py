@@ -31,26 +31,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "logL evals/sec at 256² px, batched+emulated vs plain GPU, |ΔlogL|≤0.01"
-# algorithmic work per evaluation of this workload, counted from the
-# reference formulas (add/mul/div = 1, FMA = 2, transcendental = 1):
-# 165 flops per ray traced (49 rays per pixel), PSF 441 MAC per pixel,
-# chi^2 4 flops
-FLOP_CONV = 2 * 441 * 65536
-FLOP_CHI2 = 4 * 65536
-PEAK_FP32 = 157.3  # TFLOP/s, MI355X published FP32 (packed / matrix) peak
-PEAK_FP32_NONPACKED = 78.6
+PEAK_FP32_NONPACKED = 78.6  # TFLOP/s: one wave64 FMA per CU per cycle (packed FP32 is not used)
 PEAK_HBM = 8000.0  # GB/s
 # The render kernel is bound by VALU issue: one wave64 VALU instruction per
-# CU per cycle (256 CUs at 2.4 GHz).  Its C3 ray (tabulated EPL + shear in the
-# lens-frame form, Gaussian source) is 35 VALU instructions in the gfx950 ISA
-# of the generated program (rocprofv3 SQ_INSTS_VALU per launch agrees)
-RAY_VALU = 35
-VALU_ISSUE_PER_S = 256 * 2.4e9  # wave instructions per second
-# HBM bytes per 4096-candidate render launch (prior draws, pixel cull on),
-# measured with rocprofv3 --pmc: FETCH_SIZE 34190 KiB (x 2, the factor a
-# 1 GiB copy calibrates) in and WRITE_SIZE 236470 KiB (x 1) out, the lit
-# 8 x 8 squares of the model images (the cull pass writes the dark ones)
-TRAFFIC_RENDER_4096 = (34190 * 2 + 236470) * 1024
+# CU per cycle, 256 CUs at 2.4 GHz
+VALU_ISSUE_PEAK = 256 * 2.4e9  # wave instructions per second
+# reference-formula work of this workload (add/mul/div 1, FMA 2,
+# transcendental 1): 165 flop per ray, 49 rays per pixel
+REF_FLOP_PER_RAY = 165
+PROFILES = os.path.join(ROOT, "profiles")
+RENDER_PROFILE = "lensed_render_c3_counters.md"
+CONV_PROFILE = "lensed_conv_chi2_c3_counters.md"
+
+
+def profile_figure(name, key):
+    """A figure a profiles/ summary records as a line `key: value` (None if
+    the summary or the line is missing)."""
+    try:
+        with open(os.path.join(PROFILES, name)) as f:
+            for line in f:
+                if line.strip().startswith(key + ":"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def make_workload(K):
@@ -153,23 +157,53 @@ def kernel_times(wl, K, launches, flags=0):
 
 
 def render_roofline(kt, tr, K):
-    """The render kernel against its bound, VALU issue, over the rays it
-    traced (the pixel cull skips the rest, whose values are exactly 0)."""
-    rays = tr["waves_traced"] * 64 * 49
+    """The render kernel against its bound, VALU issue.  Its VALU wave
+    instructions per traced 8 x 8 square (49 rays per pixel) come from the
+    rocprofv3 SQ_INSTS_VALU count of the same launch, recorded in profiles/;
+    the squares traced and the kernel's time from this run."""
+    waves = tr["waves_traced"]
+    rays = waves * 64 * 49
     s = kt["render_kernel"] / 1e3
-    achieved = 165 * rays / s / 1e12
-    return {"bound": "valu", "kernel": "lensed_render", "achieved": round(achieved, 3), "peak": PEAK_FP32,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
-            "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED, 4),
-            "traffic": int(TRAFFIC_RENDER_4096 * K / 4096), "traffic_unit": "bytes per launch (rocprofv3 --pmc)",
-            "algorithmic_flop_per_launch": 165 * rays, "rays_traced_per_launch": rays,
-            "rays_traced_frac": round(tr["rays_traced_frac"], 4),
+    per_sq = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_traced_square")
+    busy = profile_figure(RENDER_PROFILE, "valu_busy_fraction")
+    traffic = profile_figure(RENDER_PROFILE, "hbm_bytes_per_launch")
+    achieved = per_sq * waves / s / 1e9 if per_sq else None
+    peak = VALU_ISSUE_PEAK / 1e9
+    return {"bound": "valu", "kernel": "lensed_render", "unit": "G wave-instructions/s",
+            "achieved": None if achieved is None else round(achieved, 2), "peak": peak,
+            "frac": None if achieved is None else round(achieved / peak, 4),
+            "traffic": None if traffic is None else int(traffic * K / 4096),
+            "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, calibrated)",
             "avg_launch_ms": round(kt["render_kernel"], 4), "cull_pass_ms": round(kt["cull_pass"], 4),
-            "valu_instr_per_ray": RAY_VALU,
-            "valu_issue_frac": round(RAY_VALU * rays / 64 / VALU_ISSUE_PER_S / s, 4),
-            "note": "over the rays the kernel traces (165 flop each by the reference formulas, the "
-                    "tenth-order EPL series); the table path does that work in 35 VALU instructions per ray, "
-                    "so frac can exceed 1: valu_issue_frac is the kernel's position against its actual bound"}
+            "valu_wave_instructions_per_traced_square": per_sq, "squares_traced": waves,
+            "rays_traced_per_launch": rays, "rays_traced_frac": round(tr["rays_traced_frac"], 4),
+            "valu_busy_fraction_from_profile": busy,
+            "reference_flop_equivalent": {
+                "tflops": round(REF_FLOP_PER_RAY * rays / s / 1e12, 2),
+                "note": "the reference formulas' 165 flop per traced ray (its tenth-order series); the table "
+                        "path computes the same ray in fewer instructions, so this rate is not a fraction of "
+                        "any hardware peak"},
+            "note": "frac = VALU wave instructions issued / (256 CUs x 2.4 GHz x kernel time); transcendentals "
+                    "take two issue slots, so the instruction count tops out below 1 (busy fraction from "
+                    "SQ_ACTIVE_INST_VALU in the profile)"}
+
+
+def conv_roofline(kt, K, npix=65536, taps=21):
+    """The convolution + chi^2 kernel against both its bounds: the rank-1
+    separable passes' FP32 work and the model bytes it must read."""
+    s = kt["convolve"] / 1e3
+    flop = (2 * 2 * taps + 4) * npix * K  # row pass, column pass, w (m - d)^2
+    model_bytes = 4 * npix * K
+    fetch = profile_figure(CONV_PROFILE, "hbm_bytes_per_launch")
+    hbm = None if fetch is None else fetch * K / 4096
+    return {"kernel": "lensed_conv_chi2", "avg_launch_ms": round(kt["convolve"], 4),
+            "algorithmic_tflops": round(flop / s / 1e12, 3),
+            "frac_fp32_nonpacked": round(flop / s / 1e12 / PEAK_FP32_NONPACKED, 4),
+            "model_bytes_per_launch": model_bytes,
+            "hbm_bytes_per_launch": None if hbm is None else int(hbm),
+            "hbm_over_model_bytes": None if hbm is None else round(hbm / model_bytes, 3),
+            "achieved_gbps": None if hbm is None else round(hbm / s / 1e9, 1),
+            "frac_hbm": None if hbm is None else round(hbm / s / 1e9 / PEAK_HBM, 4)}
 
 
 def near_truth(wl, pids, n, scale=2e-4, seed=7):
@@ -202,30 +236,91 @@ def accuracy(wl, pids, K, flags):
             "max_rel_dlogL_prior_draws": float(np.max(np.abs(loc - lc) / np.abs(lc)))}
 
 
-def cpu_baseline(wl, pids, seconds):
-    """The oracle's fast FP32 build (render + convolve + chi^2 per candidate,
-    OpenMP over pixels) on a bounded sample of the same candidates."""
+def cpu_baseline(wl, pids, n=20, warm=3):
+    """The CPU baseline: the oracle's fast FP32 build (render + convolve +
+    chi^2, OpenMP over pixels, -O3 -ffast-math) timed per logL at C3 (this
+    workload) and C1, median over n evaluations of prior draws after `warm`,
+    on all host threads and on one."""
     from lensed_amd import quadrature as Q
+    from lensed_amd import workloads as W
     from oracle import oracle as O
     qq, ww = Q.rule("g3k7")
-    rng = np.random.default_rng(wl.cand_seed)
-    lo_hi = [wl.prior_range(k) for k in pids]
-    done, t0 = 0, time.perf_counter()
-    image, weight = wl.image, wl.weight
-    while True:
-        u = rng.random(len(pids))
-        p = np.array([a + x * (b - a) for x, (a, b) in zip(u, lo_hi)], np.float32)
-        v, _ = O.render(wl.objects, p, wl.shape, qq, ww, dtype=np.float32, fast=True)
-        m = O.convolve(v, wl.psf, dtype=np.float32, fast=True)
-        O.chi2_f32terms(image, weight, m)
-        done += 1
-        dt = time.perf_counter() - t0
-        if dt >= seconds and done >= 3:
-            break
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": done / dt, "unit": "logL evals/s", "cores": threads, "kind": "port",
-            "sample": f"{done} candidates of the same workload from the same priors, one at a time "
-                      f"({dt:.1f} s): oracle/liboracle_fast.so FP32, -O3 -ffast-math, OpenMP over pixels"}
+
+    def draws_of(w, keys, count, seed):
+        rng = np.random.default_rng(seed)
+        rngs = [w.prior_range(k) for k in keys]
+        return [np.array([a + x * (b - a) for x, (a, b) in zip(rng.random(len(keys)), rngs)], np.float32)
+                for _ in range(count)]
+
+    def latency(w, keys, nthreads):
+        O.set_threads(nthreads)
+        times = []
+        for i, p in enumerate(draws_of(w, keys, warm + n, w.cand_seed)):
+            t0 = time.perf_counter()
+            v, _ = O.render(w.objects, p, w.shape, qq, ww, dtype=np.float32, fast=True)
+            m = O.convolve(v, w.psf, dtype=np.float32, fast=True) if w.psf is not None else v
+            O.chi2_f32terms(w.image, w.weight, m)
+            if i >= warm:
+                times.append(time.perf_counter() - t0)
+        return float(np.median(times))
+
+    c1 = W.c1()
+    c1_ids = [f"{o}.{p}" for o, nm in c1.objects for p in O.PARAMS[nm]]
+    pts, wht, err = Q.table("g3k7")
+    c1.make_data(lambda objs, p, shape, psf: O.render(objs, p, shape, pts, np.stack([wht, err], 1))[0],
+                 c1_ids, mag_keys=("source.mag",))
+    t = {"c3_all": latency(wl, pids, threads), "c3_one": latency(wl, pids, 1),
+         "c1_all": latency(c1, c1_ids, threads), "c1_one": latency(c1, c1_ids, 1)}
+    O.set_threads(threads)
+    return {"value": round(1.0 / t["c3_all"], 3), "unit": "logL evals/s", "cores": threads, "kind": "port",
+           "sample": f"C3 (this workload) and C1, {n} evaluations of prior draws each after {warm} warm-up, "
+                     "median latency; oracle/liboracle_fast.so FP32 (-O3 -ffast-math, OpenMP over pixels), "
+                     "one candidate at a time",
+           "median_ms_per_logL": {"C3_all_threads": round(t["c3_all"] * 1e3, 2),
+                                  "C3_one_thread": round(t["c3_one"] * 1e3, 2),
+                                  "C1_all_threads": round(t["c1_all"] * 1e3, 3),
+                                  "C1_one_thread": round(t["c1_one"] * 1e3, 3)},
+           "value_one_thread": round(1.0 / t["c3_one"], 3)}
+
+
+def oracle_accuracy(wl, pids, K, flags):
+    """The FP64 oracle (the checker) against the GPU path on the timed
+    launch's own candidates: 8 seeded picks of its prior draws and the
+    truth, model images and logL."""
+    from lensed_amd import quadrature as Q
+    from lensed_amd._lib import LENSED_EMULATE
+    from lensed_amd.lensed import Context
+    from oracle import oracle as O
+    emu = EMULATOR if flags & LENSED_EMULATE else None
+    pts, wht, err = Q.table("g3k7")
+    with Context(wl.spec(), max_batch=K, flags=flags, emulator=emu) as ctx:
+        cubes = wl.candidates(ctx.ndims(), K)
+        lnew, phys_s = ctx.loglike_batch(cubes)
+        phys = np.zeros((K, len(pids)))
+        phys[:, ctx.pmap()] = phys_s
+        picks = list(np.sort(np.random.default_rng(wl.cand_seed + 7).choice(K, 8, replace=False)))
+        truth = wl.truth_vector(pids)
+        lt = ctx.loglike_phys(truth[None])[0]
+        rows = []
+        for i, (p, lg) in enumerate([(phys[j], lnew[j]) for j in picks] + [(truth, lt)]):
+            img = ctx.render(p)["image"]
+            pcs = [1.0 + 0.5 * (wl.psf.shape[1] % 2 == 0), 1.0 + 0.5 * (wl.psf.shape[0] % 2 == 0), 1.0, 1.0]
+            v, _ = O.render(wl.objects, p, wl.shape, pts, np.stack([wht, err], 1), pcs)
+            m64 = O.convolve(v, wl.psf)
+            l64 = -0.5 * O.chi2_d(wl.image, wl.weight, m64)
+            peak = np.abs(m64).max()
+            mask = np.abs(m64) > 1e-3 * peak
+            rel = float(np.max(np.abs(img - m64)[mask] / np.abs(m64[mask]))) if mask.any() else 0.0
+            rows.append((rel, abs(lg - l64), abs(lg - l64) / max(1.0, abs(l64)), i == 8))
+    acc = {"candidates": f"8 seeded picks of the timed launch's {K} prior draws, and the truth",
+           "model_max_rel_err": max(r[0] for r in rows),
+           "max_abs_dlogL_prior_draws": max(r[1] for r in rows if not r[3]),
+           "max_rel_dlogL_prior_draws": max(r[2] for r in rows if not r[3]),
+           "abs_dlogL_truth": [r[1] for r in rows if r[3]][0],
+           "bars": "model 1e-4 relative where above 1e-3 of its peak; |dlogL| 1e-2 near the truth, "
+                   "1e-2 + 1e-5 |logL| for prior draws (FP32 model rounding at |logL| up to 1e7)"}
+    return acc
 
 
 def nested_compare(wl, pids, K, nlive=400):
@@ -295,7 +390,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--plain", type=int, default=1000, help="candidates timed on the plain path")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-nested", action="store_true", help="skip the end-to-end nested-sampling comparison")
     ap.add_argument("--value-only", action="store_true",
                     help="time only the batched + emulated launches and the roofline (for profiler runs)")
@@ -343,9 +437,11 @@ def main():
     same = bool(np.array_equal(lplain, lnew[: len(lplain)]))
     kt, tr = kernel_times(wl, K, 3, LENSED_EMULATE)
     kt_exact, _ = kernel_times(wl, K, 2)
-    conv_s = kt["convolve"] / 1e3
-    cpu = cpu_baseline(wl, pids, args.cpu_seconds)
     nest = None if args.no_nested else nested_compare(wl, pids, K)
+    # the CPU leg, last: the oracle as the checker (strict IEEE build), then
+    # as the baseline (its fast-math build sets flush-to-zero in the process)
+    acc64 = oracle_accuracy(wl, pids, K, LENSED_EMULATE)
+    cpu = cpu_baseline(wl, pids)
 
     out = {
         "metric": METRIC,
@@ -387,8 +483,8 @@ def main():
         "stages_note": "set_params includes the emulator, held-out check and fallback table kernels; render is "
                        "cull_pass + render_kernel",
         "stages_ms_per_launch_exact": {k: round(v, 4) for k, v in kt_exact.items()},
-        "conv_chi2": {"achieved_tflops": round((FLOP_CONV + FLOP_CHI2) * K / conv_s / 1e12, 3),
-                      "avg_launch_ms": round(kt["convolve"], 4)},
+        "conv_chi2": conv_roofline(kt, K),
+        "accuracy_vs_fp64_oracle": acc64,
         "cpu_baseline": cpu,
         "nested_sampling": nest,
     }

@@ -137,7 +137,9 @@ enum {
     LENSED_EMULATE = 2, /* EPL angular factor from the trained emulator table */
     LENSED_TABLE = 4,   /* EPL angular factor from the exact-series table */
     LENSED_GRAPH = 8,   /* replay batched launches from a captured hipGraph */
-    LENSED_NO_CULL = 16 /* trace every ray (no skipping of pixels whose rays provably all miss the source) */
+    LENSED_NO_CULL = 16, /* trace every ray (no skipping of pixels whose rays provably all miss the source) */
+    LENSED_AUTOTUNE = 32 /* time the launch shapes (lensed_tuning) at the first launch of each layout and keep
+                            the fastest; choices cached on disk per device, program and layout */
 };
 
 /* One observed system: data as prepared by src/lensed.c:406-511. */
@@ -218,6 +220,39 @@ long lensed_profile(const lensed* l, double* ms8);
  * accumulated under LENSED_PROFILE (part of the render stage of
  * lensed_profile).  Waits for the launch; any output may be NULL. */
 int lensed_render_stats(const lensed* l, long* items, long* listed, long* waves, double* cull_ms);
+
+/* ----------------------------------------------------------- launch shapes */
+
+/* Launch shapes of the likelihood kernels: the knobs the autotuner searches.
+ * No counterpart in the reference, which sizes its work groups from device
+ * queries (src/lensed.c:917-948 for render, the halving loop of
+ * src/lensed.c:998-1027 for convolve).  A zero field keeps the default.  No
+ * setting changes a result: the model planes and every logL are the same
+ * bits under all of them. */
+typedef struct {
+    double list_fraction; /* the render walks the cull's list of lit 8 x 8 squares while under this fraction
+                             of all squares is lit, else every item, skipping dark squares by their flag
+                             (0.75; negative: never the list, above 1: always) */
+    double render_grid;   /* render blocks as a multiple of the resident count (1) */
+    double cull_grid;     /* pixel-cull blocks, the same (1) */
+    double conv_grid;     /* convolution blocks, the same (1) */
+    int conv_rows;        /* output rows per thread of the convolution tile (4 rows of threads): 2, 8 or 16
+                             (8, or 2 when the launch is too small to fill the device); 16 needs 64 + PSF
+                             height - 1 rows of LDS and falls back to 8 where that passes 64 KiB */
+    int conv_order;       /* convolution items: 1 candidate-fastest over the grid, 2 each block a run of one
+                             candidate's tiles down 64-column strips within its XCD's share (1) */
+    /* read back by lensed_get_tuning() after an autotuned launch, else 0: */
+    double default_ms;    /* the launch chain with the defaults, as the autotuner timed it */
+    double tuned_ms;      /* with the knobs it chose */
+} lensed_tuning;
+
+/* knobs for launches from now on (layouts already built are rebuilt);
+ * NULL restores the defaults.  Returns 0, or -1 for an invalid value. */
+int lensed_set_tuning(lensed* l, const lensed_tuning* t);
+/* the knobs of the last launch, resolved (defaults filled in; the
+ * autotuner's choice under LENSED_AUTOTUNE), or the requested ones before
+ * any launch */
+int lensed_get_tuning(const lensed* l, lensed_tuning* t);
 
 /* --------------------------------------------------------- EPL emulator */
 

@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(HERE, "liblensed_amd.so")
 OBJ_LENS, OBJ_SOURCE, OBJ_FOREGROUND = ord("L"), ord("S"), ord("F")
 (PAR_PARAMETER, PAR_POSITION_X, PAR_POSITION_Y, PAR_RADIUS, PAR_MAGNITUDE, PAR_AXIS_RATIO,
  PAR_POS_ANGLE) = range(7)
-LENSED_PROFILE, LENSED_EMULATE, LENSED_TABLE, LENSED_GRAPH, LENSED_NO_CULL = 1, 2, 4, 8, 16
+LENSED_PROFILE, LENSED_EMULATE, LENSED_TABLE, LENSED_GRAPH, LENSED_NO_CULL, LENSED_AUTOTUNE = 1, 2, 4, 8, 16, 32
 
 
 class Param(Structure):
@@ -34,6 +34,12 @@ class System(Structure):
 class Options(Structure):
     _fields_ = [("device", c_int), ("flags", c_int), ("max_batch", c_size_t), ("emulator", c_char_p),
                 ("emulator_tol", c_double)]
+
+
+class Tuning(Structure):
+    _fields_ = [("list_fraction", c_double), ("render_grid", c_double), ("cull_grid", c_double),
+                ("conv_grid", c_double), ("conv_rows", c_int), ("conv_order", c_int), ("default_ms", c_double),
+                ("tuned_ms", c_double)]
 
 
 class NsOptions(Structure):
@@ -116,6 +122,8 @@ SIGNATURES = {
                                      POINTER(c_float), POINTER(c_float)]),
     "lensed_emulator_stats": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
     "lensed_render_stats": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
+    "lensed_set_tuning": (c_int, [c_void_p, POINTER(Tuning)]),
+    "lensed_get_tuning": (c_int, [c_void_p, POINTER(Tuning)]),
     "lensed_fits_read": (c_int, [c_char_p, POINTER(c_size_t), POINTER(c_size_t), POINTER(POINTER(c_float)),
                                  POINTER(c_double)]),
     "lensed_fits_free": (None, [c_void_p]),

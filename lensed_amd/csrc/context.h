@@ -69,6 +69,11 @@ struct Layout {
     bool conv_small = false;        // short tiles: launch too small to fill the GPU
     bool any_cull = false;          // some system culls pixels before the render (lensed_cull)
     int grid_conv = 0, grid_render = 0;  // resident blocks for this launch's LDS use
+    int grid_cull = 0;
+    int list_max = 0;               // the render walks the cull's list below this many lit squares
+    int conv_rows = 8;              // rows per thread of the convolution tile (2, 8, 16)
+    int conv_order = 0;             // convolution item order (kernel argument, 0 or 1)
+    lensed_tuning tune{};           // the resolved knobs (what lensed_get_tuning reports)
     lensed_desc* d_desc = nullptr;
     hipGraphExec_t graph = nullptr;
     int id = 0;                     // tells the launches of this layout apart on the device (owner)
@@ -88,7 +93,8 @@ struct lensed {
     std::vector<lsd::System> sys;
     hipModule_t module = nullptr;
     hipFunction_t f_set = nullptr, f_render = nullptr, f_conv = nullptr, f_conv_small = nullptr, f_final = nullptr,
-                  f_tables = nullptr, f_cull = nullptr;
+                  f_tables = nullptr, f_cull = nullptr, f_conv_tall = nullptr;
+    lensed_tuning tune_req{};    // requested launch shapes (zero fields: defaults)
     int cu_count = 1;
     hipStream_t stream = nullptr;
     int grid_render = 0, grid_conv = 0, grid_conv_small = 0;

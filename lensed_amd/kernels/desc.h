@@ -10,8 +10,10 @@
 #define LENSED_CONV_TY 4      // threads along y
 #define LENSED_CONV_RY 8      // output rows per thread, large launches (a multiple of 8)
 #define LENSED_CONV_RY_SMALL 2  // output rows per thread, launches too small to fill the GPU
+#define LENSED_CONV_RY_TALL 16  // output rows per thread of the 64-row tile (a tuning choice)
 // chi^2 partial sums are formed over fixed 8-row x 64-column blocks, so the
-// short tile (4 x 2 rows) is exactly one block and the tall one (4 x 8) four
+// short tile (4 x 2 rows) is exactly one block, the default one (4 x 8) four
+// and the tall one (4 x 16) eight
 #define LENSED_CONV_BS (LENSED_CONV_TX * LENSED_CONV_TY)
 #define LENSED_PSF_MAX 63     // largest PSF side the conv tile is sized for
 #define LENSED_EPL_NT 256     // EPL angular table: intervals of the diamond angle p in [0, 1]

@@ -192,18 +192,19 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const u32* __restrict__ blocks, const float4* __restrict__ quad, const float4* __restrict__ tables,
     const float4* __restrict__ npre, const int* __restrict__ live, const int* __restrict__ trace,
-    float* __restrict__ model, float* __restrict__ errp)
+    float* __restrict__ model, float* __restrict__ errp, int list_max)
 {
     extern __shared__ float4 tab_lds[];
-    // after a cull pass that left most squares dark, only the lit 8 x 8
-    // squares it listed, four to an entry, all four of one candidate
-    // (live[1..live[0]]: square = item * 4 + w, -1 for none); otherwise
-    // every item, square w to wave w, a culled system's dark squares (zeroed
-    // by the cull) skipped by their flag.  (Walking a list that holds nearly
-    // every square measured much slower than walking the items: 304 against
-    // 221 ms per launch of 64 systems whose sources never reach exact zero.)
+    // after a cull pass that left fewer than list_max squares lit, only the
+    // lit 8 x 8 squares it listed, four to an entry, all four of one
+    // candidate (live[1..live[0]]: square = item * 4 + w, -1 for none);
+    // otherwise every item, square w to wave w, a culled system's dark
+    // squares (zeroed by the cull) skipped by their flag.  (Walking a list
+    // that holds nearly every square measured much slower than walking the
+    // items: 304 against 221 ms per launch of 64 systems whose sources never
+    // reach exact zero; list_max is three squares in four by default.)
     const int wave = (int)threadIdx.x >> 6;
-    const bool listed = live && live[0] < 3 * nitems;  // under three lit squares in four
+    const bool listed = live && live[0] < list_max;
     int first, end;
     lensed_xcd_run(listed ? live[0] / 4 : nitems, first, end);
     int staged = -1;  // first item of the candidate whose table is in LDS
@@ -816,12 +817,19 @@ __device__ inline void lensed_conv_item(const lensed_desc& d, int cand, int t,
     (void)red;
 }
 
+// Two item orders.  order 0: candidate-fastest, items strided over the
+// persistent grid, so every block sees a mix of tiles (dark tiles after a
+// pixel cull cost less).  order 1: tile-fastest within a candidate, down
+// each 64-column strip of the image, and each block a contiguous run of its
+// XCD's eighth of the items (lensed_xcd_run): consecutive tiles of a block
+// share their PSF halo rows, which the previous tile just brought into the
+// XCD's L2.
 template <int RY>
 __device__ inline void lensed_conv_chi2_body(const lensed_desc* __restrict__ sys, int nsys, int nitems,
                                              const float* __restrict__ model, const float* __restrict__ image,
                                              const float* __restrict__ weight, const float* __restrict__ psf,
                                              double* __restrict__ partials, float* __restrict__ conv_out,
-                                             float* __restrict__ chi2_out, const int* __restrict__ trace)
+                                             float* __restrict__ chi2_out, const int* __restrict__ trace, int order)
 {
     extern __shared__ float4 tile4[];
     float* tile = (float*)tile4;
@@ -830,15 +838,27 @@ __device__ inline void lensed_conv_chi2_body(const lensed_desc* __restrict__ sys
     auto run = [&](int item) {
         int s = lensed_find_sys<1>(sys, nsys, item);
         const lensed_desc& d = sys[s];
-        // candidate-fastest order: a persistent block's items spread over
-        // the image's tiles (dark tiles after a pixel cull cost less)
         int local = item - d.item_c;
-        int t = local / d.ncand;
-        int cand = local - t * d.ncand;
+        int t, cand;
+        if (order == 0) {
+            t = local / d.ncand;
+            cand = local - t * d.ncand;
+        } else {
+            cand = local / d.ntile_c;
+            const int k = local - cand * d.ntile_c, tiles_y = d.ntile_c / d.tiles_x;
+            const int tx = k / tiles_y, ty = k - tx * tiles_y;
+            t = ty * d.tiles_x + tx;
+        }
         lensed_conv_item<RY>(d, cand, t, model, image, weight, psf, partials, conv_out, chi2_out, trace, tile, red,
                              rows8);
     };
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) run(item);
+    if (order == 0) {
+        for (int item = blockIdx.x; item < nitems; item += gridDim.x) run(item);
+    } else {
+        int first, end;
+        lensed_xcd_run(nitems, first, end);
+        for (int item = first; item < end; ++item) run(item);
+    }
 }
 
 // six waves per SIMD (at most 80 VGPRs): with the rank-1 row pass over the
@@ -848,10 +868,10 @@ extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) __attribute__((amdgpu_wa
     const float* __restrict__ model, const float* __restrict__ image,
     const float* __restrict__ weight, const float* __restrict__ psf,
     double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out,
-    const int* __restrict__ trace)
+    const int* __restrict__ trace, int order)
 {
     lensed_conv_chi2_body<LENSED_CONV_RY>(sys, nsys, nitems, model, image, weight, psf, partials, conv_out,
-                                          chi2_out, trace);
+                                          chi2_out, trace, order);
 }
 
 extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) void lensed_conv_chi2_small(
@@ -859,10 +879,23 @@ extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) void lensed_conv_chi2_sm
     const float* __restrict__ model, const float* __restrict__ image,
     const float* __restrict__ weight, const float* __restrict__ psf,
     double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out,
-    const int* __restrict__ trace)
+    const int* __restrict__ trace, int order)
 {
     lensed_conv_chi2_body<LENSED_CONV_RY_SMALL>(sys, nsys, nitems, model, image, weight, psf, partials,
-                                                conv_out, chi2_out, trace);
+                                                conv_out, chi2_out, trace, order);
+}
+
+// 64-row tiles (16 rows per thread): a third less halo per output row than
+// the 32-row tile, at four waves per SIMD for the larger register tile
+extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) __attribute__((amdgpu_waves_per_eu(4))) void lensed_conv_chi2_tall(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems,
+    const float* __restrict__ model, const float* __restrict__ image,
+    const float* __restrict__ weight, const float* __restrict__ psf,
+    double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out,
+    const int* __restrict__ trace, int order)
+{
+    lensed_conv_chi2_body<LENSED_CONV_RY_TALL>(sys, nsys, nitems, model, image, weight, psf, partials,
+                                               conv_out, chi2_out, trace, order);
 }
 
 // ---------------------------------------------------------------- finalize

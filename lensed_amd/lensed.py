@@ -279,6 +279,19 @@ class Context:
                "render_stats")
         return n.value, kept.value, waves.value, ms.value
 
+    def set_tuning(self, **knobs):
+        """Launch shapes for the launches from now on (lensed_tuning fields;
+        unnamed ones keep their defaults).  Results do not depend on them."""
+        t = L.Tuning(**knobs)
+        _check(L.load().lensed_set_tuning(self.ptr, byref(t)), "set_tuning")
+
+    def tuning(self):
+        """The launch shapes of the last launch, resolved (or as requested
+        before any launch), as a dict of the lensed_tuning fields."""
+        t = L.Tuning()
+        _check(L.load().lensed_get_tuning(self.ptr, byref(t)), "get_tuning")
+        return {name: getattr(t, name) for name, _ in L.Tuning._fields_}
+
     def profile(self):
         """Accumulated stage times (ms) with the reference's stage names."""
         ms = (c_double * 8)()

@@ -58,6 +58,14 @@ int orc_type(int kind) { return kind >= 0 && kind < ORC_NKINDS ? ORC_TYPES[kind]
 #undef REAL
 #undef SFX
 
+/* threads of the OpenMP loops (the fast build; 1 in the strict one) */
+#ifdef _OPENMP
+#include <omp.h>
+int orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); return omp_get_max_threads(); }
+#else
+int orc_set_threads(int n) { (void)n; return 1; }
+#endif
+
 /* Sersic b(n) of objects/sersic.cl:32-33 in FP64, for the check against its
  * defining equation Gamma(2n, b) / Gamma(2n) = 1/2 (docs/sources.md:109-111) */
 double orc_sersic_b(double n) { return sersic_b_d(n); }

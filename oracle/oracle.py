@@ -62,6 +62,8 @@ def _load(name):
         lib.orc_chi2_f32terms.restype = c_double
         lib.orc_chi2_d.argtypes = [c_int, dp, dp, dp]
         lib.orc_chi2_d.restype = c_double
+        lib.orc_set_threads.argtypes = [c_int]
+        lib.orc_set_threads.restype = c_int
         lib.orc_sersic_b.argtypes = [c_double]
         lib.orc_sersic_b.restype = c_double
         lib.orc_gauss_quantile.argtypes = [c_double]
@@ -162,6 +164,11 @@ def chi2_f32terms(image, weight, model):
 def chi2_d(image, weight, model):
     a = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (image, weight, model)]
     return lib().orc_chi2_d(len(a[0]), *[_ptr(v, c_double) for v in a])
+
+
+def set_threads(n, fast=True):
+    """OpenMP threads of the oracle's pixel loops; returns the count in use."""
+    return lib(fast).orc_set_threads(int(n))
 
 
 def sersic_b(n):
