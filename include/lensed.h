@@ -197,6 +197,24 @@ void dumper(int* nsamples, int* nlive, int* npar, double** physlive,
 int lensed_loglike_batch(lensed* l, size_t s, const double* cubes, size_t ncand,
                          double* phys_out, double* lnew_out);
 
+/* No counterpart in the reference: lensed_loglike_batch() for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree or
+ * other allocating routine: every per-launch buffer (descriptor, parameters,
+ * object blocks, EPL tables, model images, the pixel cull's list and flags,
+ * chi^2 partials, logL) is carved from one static device array of 128 MiB
+ * per GPU, which the library holds until the process ends (its address is
+ * found once per device when a context is created).  Limits: ncand <= the
+ * context's max_batch, and everything must fit the 128 MiB (about 470
+ * candidates at 256 x 256; larger counts are rejected with an error that
+ * names the limit).  EPL tables are always built from the exact series here,
+ * also for a context created with LENSED_EMULATE.  Every call on a GPU shares
+ * its array, which keeps the last call's contents: calls that overlap on one
+ * GPU (from several threads or contexts) race and must be serialised by the
+ * caller.  Results equal lensed_loglike_batch()'s bit for bit. */
+int lensed_loglike_batch_noalloc(lensed* l, size_t s, const double* cubes, size_t ncand, double* phys_out,
+                                 double* lnew_out);
+
 /* Physical parameters in object order ([ncand][npar]) -> log-likelihoods,
  * for every system at once: phys[s], ncand[s], lnew[s]. */
 int lensed_loglike_systems(lensed* l, const double* const* phys, const size_t* ncand,

@@ -108,6 +108,8 @@ SIGNATURES = {
                       POINTER(c_double), POINTER(c_double), POINTER(c_double), c_void_p]),
     "lensed_loglike_batch": (c_int, [c_void_p, c_size_t, POINTER(c_double), c_size_t, POINTER(c_double),
                                      POINTER(c_double)]),
+    "lensed_loglike_batch_noalloc": (c_int, [c_void_p, c_size_t, POINTER(c_double), c_size_t, POINTER(c_double),
+                                             POINTER(c_double)]),
     "lensed_loglike_systems": (c_int, [c_void_p, POINTER(POINTER(c_double)), POINTER(c_size_t),
                                        POINTER(POINTER(c_double))]),
     "lensed_render": (c_int, [c_void_p, c_size_t, POINTER(c_double), POINTER(c_float), POINTER(c_float),

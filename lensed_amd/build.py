@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "liblensed_amd.so")
 ARCH = os.environ.get("LENSED_ARCH", "gfx950")
 
 HOST_SOURCES = ["capi.cpp", "config.cpp", "context.cpp", "data.cpp", "nested.cpp", "prior.cpp", "program.cpp", "psf.cpp", "quadrature.cpp", "rtc.cpp"]
-HIP_SOURCES = ["emulator.hip"]
+HIP_SOURCES = ["emulator.hip", "scratch.hip"]
 
 # packed FP32 instructions stay out of every HIP code object built here
 NO_PACKED_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -80,6 +80,25 @@ struct Layout {
     unsigned long long last_use = 0;  // context's launch clock at its last use (eviction order)
 };
 
+// device buffers of one launch: the context's own, or carved from the static
+// scratch by the entry point that allocates nothing
+struct Buffers {
+    lensed_desc* desc = nullptr;
+    float* params = nullptr;
+    unsigned* blocks = nullptr;
+    float* model = nullptr;
+    double* partials = nullptr;
+    double* logl = nullptr;
+    void* tables = nullptr;
+    void* npre = nullptr;
+    int* trace = nullptr;
+    int* live = nullptr;
+    int* owner = nullptr;
+    int* written = nullptr;
+    bool emulate = true;  // emulated EPL tables where the context has the emulator
+    bool record = true;   // the context's statistics describe this launch (lensed_render_stats)
+};
+
 // cached launch layouts per context; beyond this many the least recently
 // used one is freed (its descriptors and graph)
 constexpr size_t kMaxLayouts = 32;
@@ -137,6 +156,12 @@ struct lensed {
     int* d_written = nullptr;    // per render square: its model values may be nonzero (lensed_cull)
     int next_layout_id = 0;
     unsigned long long layout_clock = 0;
+    // the entry point that allocates nothing: this device's static scratch,
+    // a launch layout per system built at creation (patched per call) and a
+    // pinned staging slot for its descriptor
+    void* scratch = nullptr;
+    std::vector<lsd::Layout> noalloc;
+    lensed_desc* h_desc = nullptr;
     size_t cap_trace = 0;
     // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem
     hipEvent_t ev[10] = {};  // the eight stages' bounds, then the end of the cull pass
