@@ -89,8 +89,9 @@ def draws(wl, ndim, K, n):
 
 def time_batched(wl, K, steps, warmup, flags):
     """Seconds for `steps` launches of K candidates (fresh draws every
-    launch); logL of the last; the emulator's statistics when the emulated
-    path runs."""
+    launch); logL of the last; the emulator's statistics (last launch, and
+    the certification) when the emulated path runs; the launch shapes the
+    autotuner chose (at the first warm-up launch)."""
     from lensed_amd._lib import LENSED_EMULATE
     from lensed_amd.lensed import Context
     emu = EMULATOR if flags & LENSED_EMULATE else None
@@ -102,8 +103,19 @@ def time_batched(wl, K, steps, warmup, flags):
         for i in range(warmup, warmup + steps):
             lnew, _ = ctx.loglike_batch(sets[i])
         dt = time.perf_counter() - t0
-        stats = ctx.emulator_stats() if emu else None
-    return dt, lnew, stats
+        stats = (ctx.emulator_stats(), ctx.emulator_cells()) if emu else None
+        tuning = ctx.tuning()
+    return dt, lnew, stats, tuning
+
+
+def emulator_block(stats):
+    (nf, nc, _), (ncells, ncert, cert_err) = stats
+    return {"tables_from_exact_series": nf, "tables": nc, "certified_cells": ncert, "cells": ncells,
+            "largest_heldout_abs_error_vs_fp64_series": cert_err, "certification_bound": 4e-6,
+            "weights": "lensed_amd/data/epl_mlp.bin (2^22 seeded samples, 40 epochs, trained on the GPU by "
+                       "lensed_amd.train_emulator)",
+            "note": "the network builds a candidate's table only in a (q, t) cell certified at the bound "
+                    "(certified once per weights when the context is created); the exact series elsewhere"}
 
 
 def time_plain(wl, n):
@@ -131,6 +143,9 @@ def time_plain(wl, n):
     return n / dt, np.array([v.value for v in lnew])
 
 
+TUNED = 0  # flags of the timed legs (LENSED_AUTOTUNE, set in main)
+
+
 def kernel_times(wl, K, launches, flags=0):
     """Stage times (ms per launch, hipEvents on the launch stream, fresh
     draws every launch), with the render stage split into the cull pass and
@@ -139,7 +154,7 @@ def kernel_times(wl, K, launches, flags=0):
     from lensed_amd._lib import LENSED_EMULATE, LENSED_PROFILE
     from lensed_amd.lensed import Context
     emu = EMULATOR if flags & LENSED_EMULATE else None
-    with Context(wl.spec(), max_batch=K, flags=LENSED_PROFILE | flags, emulator=emu) as ctx:
+    with Context(wl.spec(), max_batch=K, flags=LENSED_PROFILE | TUNED | flags, emulator=emu) as ctx:
         sets = draws(wl, ctx.ndims(), K, launches + 1)
         ctx.loglike_batch(sets[0])
         calls0, ms0 = ctx.profile()
@@ -370,6 +385,52 @@ def nested_compare(wl, pids, K, nlive=400):
             "logz_difference_in_sigma": round(abs(rb.logz - lz.value) / math.hypot(rb.logz_err, lze.value), 3)}
 
 
+def c4_launches(launches=3, flags_extra=0):
+    """The uneven-batch workload (64 systems of 64..512 px, SIE or EPL +
+    Sersic, 15 x 15 or 31 x 31 PSFs, 256 prior draws per system, one launch):
+    ms per launch and per stage, on the table path with the render walking
+    the pixel cull's list (list_fraction 2) and walking the items (-1), and
+    without culling."""
+    from lensed_amd import workloads as W
+    from lensed_amd._lib import LENSED_NO_CULL, LENSED_PROFILE, LENSED_TABLE
+    from lensed_amd.lensed import Context, LensedObject, SystemSpec
+    wls = W.c4()
+    for wl in wls:
+        objs = [LensedObject(oid, name) for oid, name in wl.objects]
+        pids = [f"{o.id}.{p}" for o in objs for p in o.params]
+
+        def render(objects, params, shape, psf, pids=pids):
+            pri = {k: repr(float(v)) for k, v in zip(pids, params)}
+            spec = SystemSpec(objects, pri, np.zeros(shape, np.float32), np.ones(shape, np.float32), psf=psf)
+            with Context(spec, max_batch=1) as c:
+                return c.render(params)["image"].astype(np.float64)
+        wl.make_data(render, pids, mag_keys=("source.mag",))
+        wl.pids = pids
+    rng = np.random.default_rng(404)
+    phys = [np.array([[rng.uniform(*wl.prior_range(k)) for k in wl.pids] for _ in range(256)]) for wl in wls]
+    out = {}
+    for name, extra, frac in (("list", 0, 2.0), ("walk", 0, -1.0), ("no_cull", LENSED_NO_CULL, 0.0)):
+        with Context([wl.spec() for wl in wls], max_batch=256,
+                     flags=LENSED_TABLE | LENSED_PROFILE | extra | flags_extra) as ctx:
+            ctx.set_tuning(list_fraction=frac)
+            ctx.loglike_phys(phys)
+            c0, m0 = ctx.profile()
+            cull0 = ctx.render_stats()[3]
+            for _ in range(launches):
+                ctx.loglike_phys(phys)
+            c1, m1 = ctx.profile()
+            items, kept, waves, cull1 = ctx.render_stats()
+        n = c1 - c0
+        st = {k: round((m1[k] - m0[k]) / n, 3) for k in m1}
+        st["cull_pass"] = round((cull1 - cull0) / n, 3)
+        out[name] = {"ms_per_launch": round(sum(st[k] for k in m1), 3), "stages_ms": st,
+                     "render_items": items, "items_kept": kept, "waves_traced": waves}
+    npix = sum(wl.image.size for wl in wls)
+    return {"workload": f"C4: 64 systems, {npix / 1e6:.2f} Mpx, 256 prior draws each, EPL table path",
+            "launches": launches, "paths": out,
+            "evals_per_s": {k: round(64 * 256 / (v["ms_per_launch"] / 1e3), 1) for k, v in out.items()}}
+
+
 def copy_probe(mib, reps=3):
     """Plain device-to-device copies of a known size: what the profiler's
     byte counters read for them calibrates the counters."""
@@ -393,6 +454,9 @@ def main():
     ap.add_argument("--no-nested", action="store_true", help="skip the end-to-end nested-sampling comparison")
     ap.add_argument("--value-only", action="store_true",
                     help="time only the batched + emulated launches and the roofline (for profiler runs)")
+    ap.add_argument("--no-autotune", action="store_true", help="time the default launch shapes")
+    ap.add_argument("--c4", action="store_true",
+                    help="only time the uneven 64-system workload (render list, item walk, no cull) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (byte-counter calibration) and exit")
     args = ap.parse_args()
@@ -405,11 +469,16 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
-    from lensed_amd._lib import LENSED_EMULATE, LENSED_GRAPH, LENSED_TABLE
+    if args.c4:
+        print(json.dumps(c4_launches()))
+        return
+    from lensed_amd._lib import LENSED_AUTOTUNE, LENSED_EMULATE, LENSED_GRAPH, LENSED_TABLE
+    global TUNED
+    TUNED = 0 if args.no_autotune else LENSED_AUTOTUNE
     K = args.batch
     wl, pids = make_workload(K)
     if args.value_only:
-        dt, _, emu_stats = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_EMULATE)
+        dt, _, emu_stats, tuning = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_EMULATE | TUNED)
         kt, tr = kernel_times(wl, K, 3, LENSED_EMULATE)
         print(json.dumps({"metric": METRIC, "value": round(args.steps * K / dt, 2), "unit": "logL evals/s",
                           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -418,17 +487,16 @@ def main():
                           "config": {"workload": "C3 (see the full run)", "candidates_per_launch": K,
                                      "legs": "batched + emulated only"},
                           "roofline": render_roofline(kt, tr, K), "render_trace": tr,
-                          "emulator": {"fallbacks": emu_stats[0], "checked": emu_stats[1],
-                                       "max_heldout_abs_error_vs_fp64_series": emu_stats[2]},
+                          "emulator": emulator_block(emu_stats), "launch_shapes": tuning,
                           "stages_ms_per_launch": {k: round(v, 4) for k, v in kt.items()},
                           "cpu_baseline": None}))
         return
 
-    dt_exact, lnew, _ = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH)
+    dt_exact, lnew, _, _ = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | TUNED)
     exact_rate = args.steps * K / dt_exact
-    dt_table, _, _ = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_TABLE)
+    dt_table, _, _, _ = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_TABLE | TUNED)
     table_rate = args.steps * K / dt_table
-    dt, _, emu_stats = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_EMULATE)
+    dt, _, emu_stats, tuning = time_batched(wl, K, args.steps, args.warmup, LENSED_GRAPH | LENSED_EMULATE | TUNED)
     value = args.steps * K / dt
     acc = accuracy(wl, pids, K, LENSED_EMULATE)
     acc_table = accuracy(wl, pids, K, LENSED_TABLE)
@@ -459,14 +527,12 @@ def main():
         "config": {"workload": "C3: 256x256 px, epl_plus_shear + gauss (sigma 2 px), 21x21 Gaussian PSF "
                                "FWHM 3 px, g3k7 49 rays/px, sigma 0.05, candidates uniform from the priors",
                    "candidates_per_launch": K,
-                   "path": "batched + emulated: EPL angular-factor tables from the BF16-MFMA emulator, checked "
-                           "per candidate against the FP64 series at every interval midpoint, exact-series "
-                           "fallback where the error exceeds 4e-6; hipGraph replay",
+                   "path": "batched + emulated: EPL angular-factor tables from the BF16-MFMA emulator in the "
+                           "(q, t) cells its certification passed at 4e-6 held-out error, from the exact series "
+                           "elsewhere; pixel cull; launch shapes from the autotuner; hipGraph replay",
                    "seeds": {"noise": wl.seed, "candidates": wl.cand_seed}},
-        "emulator": {"fallbacks": emu_stats[0], "checked": emu_stats[1],
-                     "max_heldout_abs_error_vs_fp64_series": emu_stats[2], "bound": 4e-6,
-                     "weights": "lensed_amd/data/epl_mlp.bin (2^22 seeded samples, 40 epochs, trained on the "
-                                "GPU by lensed_amd.train_emulator)"},
+        "emulator": emulator_block(emu_stats),
+        "launch_shapes": tuning,
         "batched_table": {"value": round(table_rate, 2), "unit": "logL evals/s",
                           "path": "batched, EPL angular factor from the per-candidate FP64-built table",
                           "accuracy_vs_exact": acc_table},
@@ -480,7 +546,7 @@ def main():
         "roofline": render_roofline(kt, tr, K),
         "render_trace": tr,
         "stages_ms_per_launch": {k: round(v, 4) for k, v in kt.items()},
-        "stages_note": "set_params includes the emulator, held-out check and fallback table kernels; render is "
+        "stages_note": "set_params includes the emulator and exact-series table kernels; render is "
                        "cull_pass + render_kernel",
         "stages_ms_per_launch_exact": {k: round(v, 4) for k, v in kt_exact.items()},
         "conv_chi2": conv_roofline(kt, K),

@@ -134,7 +134,8 @@ typedef struct lensed lensed;
 /* flags for lensed_config.flags */
 enum {
     LENSED_PROFILE = 1, /* per-stage hipEvent timing, the --profile stages */
-    LENSED_EMULATE = 2, /* EPL angular factor from the trained emulator table */
+    LENSED_EMULATE = 2, /* EPL angular factor tables from the trained emulator where it is certified, else
+                           from the exact series (LENSED_TABLE) */
     LENSED_TABLE = 4,   /* EPL angular factor from the exact-series table */
     LENSED_GRAPH = 8,   /* replay batched launches from a captured hipGraph */
     LENSED_NO_CULL = 16, /* trace every ray (no skipping of pixels whose rays provably all miss the source) */
@@ -160,7 +161,7 @@ typedef struct {
     int flags;            /* LENSED_* */
     size_t max_batch;     /* largest number of candidates per call */
     const char* emulator; /* weights file for LENSED_EMULATE, or NULL */
-    double emulator_tol;  /* held-out bound on the emulated angular factor, 0 = 4e-6 */
+    double emulator_tol;  /* certification bound on the emulated angular factor's held-out error, 0 = 4e-6 */
 } lensed_options;
 
 /* Build the context for one or more systems (several systems are evaluated
@@ -289,9 +290,25 @@ double lensed_emulator_loss_grad(const float* weights, const float* X, const flo
  * BF16-MFMA inference kernel (outputs Y[n][2] and dY[n][2]) */
 int lensed_emulator_eval(const float* weights, const float* X, const float* dX, int n, int bf16, float* Y,
                          float* dY);
-/* candidates of the last emulated launch whose held-out check failed (and
- * fell back to the exact series), candidates checked, largest held-out error */
+/* Of the last launch with EPL tables under LENSED_EMULATE: the candidates
+ * whose table the exact series built (outside the certified cells), the
+ * candidates with a table, and the largest held-out error of the
+ * certification.  No counterpart in the reference. */
 int lensed_emulator_stats(const lensed* l, long* fallbacks, long* checked, double* max_err);
+/* The certification, done once when the context is created: the network's
+ * tables at the corners and centre of every (q, t) cell of a 32 x 32 grid
+ * over q in [0.3, 1], t in [0.5, 1.5], checked against the FP64 series at
+ * every interval midpoint; a cell is certified when all five are within
+ * emulator_tol, and only candidates in certified cells take the network's
+ * table.  Cells, certified cells, largest held-out error.  No counterpart in
+ * the reference. */
+int lensed_emulator_cells(const lensed* l, long* ncells, long* ncertified, double* max_err);
+/* The exact recheck of lensed_nested_run(): candidates of an emulated launch
+ * that may be accepted (logL above the current threshold) are evaluated
+ * again with exact-series tables and take that logL.  Counts since the
+ * context was created: rechecked, changed by it, largest |delta logL|.  No
+ * counterpart in the reference. */
+int lensed_emulator_rechecks(const lensed* l, long* rechecked, long* changed, double* max_abs_dlogl);
 
 /* ------------------------------------------------------ nested sampling */
 

@@ -5,7 +5,9 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int, c_lo
                     c_void_p)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblensed_amd.so")
+# LENSED_LIB selects another build of the library (the sanitized host build
+# of the CPU fuzz tests)
+LIB_PATH = os.environ.get("LENSED_LIB") or os.path.join(HERE, "liblensed_amd.so")
 
 OBJ_LENS, OBJ_SOURCE, OBJ_FOREGROUND = ord("L"), ord("S"), ord("F")
 (PAR_PARAMETER, PAR_POSITION_X, PAR_POSITION_Y, PAR_RADIUS, PAR_MAGNITUDE, PAR_AXIS_RATIO,
@@ -123,6 +125,8 @@ SIGNATURES = {
     "lensed_emulator_eval": (c_int, [POINTER(c_float), POINTER(c_float), POINTER(c_float), c_int, c_int,
                                      POINTER(c_float), POINTER(c_float)]),
     "lensed_emulator_stats": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
+    "lensed_emulator_cells": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
+    "lensed_emulator_rechecks": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
     "lensed_render_stats": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
     "lensed_set_tuning": (c_int, [c_void_p, POINTER(Tuning)]),
     "lensed_get_tuning": (c_int, [c_void_p, POINTER(Tuning)]),

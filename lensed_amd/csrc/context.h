@@ -57,6 +57,10 @@ struct System {
     // dumper results (object order)
     std::vector<double> mean, sigma, ml, map;
     double logev = 0, logev_err = 0, logev_ins = 0, max_loglike = 0;
+    // the candidates of the last lensed_loglike_batch() call whose EPL table
+    // the emulator built (1) rather than the exact series (0); empty when
+    // no table came from the network
+    std::vector<char> emu_used;
 };
 
 // launch layout for a given number of candidates per system
@@ -77,6 +81,8 @@ struct Layout {
     lensed_desc* d_desc = nullptr;
     hipGraphExec_t graph = nullptr;
     int id = 0;                     // tells the launches of this layout apart on the device (owner)
+    bool emulate = true;            // EPL tables from the emulator where it is certified (else exact)
+    bool tuned = false;             // the autotuner has set this layout's shapes
     unsigned long long last_use = 0;  // context's launch clock at its last use (eviction order)
 };
 
@@ -131,13 +137,20 @@ struct lensed {
     double* d_partials = nullptr;
     double* d_logl = nullptr;
     void* d_tables = nullptr;   // EPL angular tables, float4 [candidates][LENSED_EPL_TABLE_F4]
-    // EPL emulator: weights (fp32 and packed bf16), table nodes, fallback flags
+    // EPL emulator: weights (fp32 and packed bf16), table nodes, per-candidate
+    // choice (1 = exact series), the certified (q, t) cells, and the choice
+    // of the last launch on the host
     void* emu_wf = nullptr;
     void* emu_wp = nullptr;
     void* d_nodes = nullptr;
     int* d_flags = nullptr;
-    float* d_errs = nullptr;
+    int* d_cells = nullptr;
+    int* h_flags = nullptr;
     float emu_tol = 4e-6f;
+    int emu_ncert = 0;            // certified cells (0: every table comes from the exact series)
+    double emu_cert_err = 0;      // largest held-out error of the certification
+    long emu_rechecked = 0, emu_changed = 0;  // exact re-evaluations of emulated candidates
+    double emu_recheck_max = 0;   // largest |logL(emulated) - logL(exact)| among them
     const lsd::Layout* last_table_layout = nullptr;
     const lsd::Layout* last_layout = nullptr;  // of the last launch (lensed_render_stats)
     // pinned host staging
@@ -174,6 +187,10 @@ namespace lsd {
 
 // evaluate ncand[s] parameter vectors (object order, float) per system
 int evaluate(::lensed* l, const std::vector<size_t>& ncand, const std::vector<const double*>& phys,
-             const std::vector<double*>& lnew);
+             const std::vector<double*>& lnew, bool emulate = true);
+
+// exact logL for the emulated candidates of system s's last batch that lie
+// above lmin (phys in sampler order, ll replaced in place); 0 on success
+int recheck_emulated(::lensed* l, size_t s, const double* phys, int n, double lmin, double* ll);
 
 }  // namespace lsd

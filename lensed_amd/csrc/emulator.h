@@ -33,5 +33,10 @@ constexpr float kT0 = 1.0f, kTS = 0.5f;    // t in [0.5, 1.5]
 // bf16 copies of W2 and W3 in MFMA operand order (see emulator.hip)
 constexpr int kPackedWords = 2 * kHidden * kHidden / 2;  // uint32 words (2 bf16 each)
 
+// certification grid over the training domain: kCellsQ x kCellsT cells of
+// (q, t); the network builds a candidate's table only in a certified cell
+constexpr int kCellsQ = 32;
+constexpr int kCellsT = 32;
+
 }  // namespace emu
 }  // namespace lsd

@@ -1,7 +1,9 @@
 // EPL emulator kernels: training data from the exact series, FP32 training
 // (forward, backward, Adam), BF16-MFMA inference of the angular-factor table
-// nodes with FP32 accumulation, and the held-out check that decides the
-// exact fallback per candidate.  Network and weight layout: emulator.h.
+// nodes with FP32 accumulation, the held-out check that certifies the
+// network per (q, t) cell once per weights, and the per-candidate choice
+// between the network and the exact series.  Network and weight layout:
+// emulator.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -269,20 +271,21 @@ static void backward(hipStream_t st, Net& net) {
 }
 
 // ------------------------------------------------------ BF16 MFMA inference
-// v_mfma_f32_32x32x8bf16_1k, wave64: A (32 x 8) lane l holds A[l%32][4(l/32)+0..3],
-// B (8 x 32) lane l holds B[4(l/32)+0..3][l%32], D (32 x 32) lane l register v
-// holds D[8(v/4) + 4(l/32) + v%4][l%32].  A layer computes Z^T = W H^T for 32
-// samples per wave: M = 128 outputs in 4 tiles, K = 128 in 16 steps.  The
-// accumulator of output tile mt holds exactly the B operand of k-steps
-// 4 mt .. 4 mt + 3 of the next layer, so activations never leave registers.
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+// gfx950's v_mfma_f32_32x32x16_bf16, wave64: A (32 x 16) lane l holds
+// A[l % 32][8 (l / 32) + 0..7], B (16 x 32) lane l holds B[8 (l / 32) + 0..7][l % 32],
+// D (32 x 32) lane l register v holds D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+// A layer computes Z^T = W H^T for 32 samples per wave: M = 128 outputs in
+// 4 tiles, K = 128 inputs in 8 steps of 16.  The K order within a step is
+// permuted so that a layer's accumulators are the next layer's B operands
+// as they stand: element i of K step S held by lane half h is input unit
+// 16 S + 8 (i / 4) + 4 h + i % 4, which is where output tile S / 2 left
+// that unit (register 8 (S % 2) + i of the lane); the weights are packed in
+// the same order, so the products are those of the plain order.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-__device__ inline short f2bf(float f) {
-    unsigned u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-    return (short)(u >> 16);
-}
+// bf16x8 operands per packed layer: 4 M-tiles x 8 K steps x 64 lanes
+constexpr int kLayerOperands = 4 * 8 * 64;
 
 __device__ inline float silu_d(float z, float& dsilu) {
     // the hardware reciprocal (about 1 ulp) rather than an IEEE division,
@@ -293,50 +296,50 @@ __device__ inline float silu_d(float z, float& dsilu) {
 }
 
 // Output rows of an MFMA 32x32 tile held by a lane: element v of M-tile mt
-// is unit 32 mt + 8 (v / 4) + 4 (lane / 32) + v % 4 of sample lane % 32.  The
-// four elements 4 gq .. 4 gq + 3 of M-tile mt are exactly what the lane holds
-// of the next layer's B operand at K step 4 mt + gq, so a layer's output is
-// packed into the next layer's operands one M-tile at a time, and only the
-// operands (never a whole layer of FP32 activations) stay live: the kernel
-// then fits several waves per SIMD to hide the weight loads.
+// is unit 32 mt + 8 (v / 4) + 4 (lane / 32) + v % 4 of sample lane % 32.  Its
+// elements 0..7 and 8..15 are the lane's B operands of K steps 2 mt and
+// 2 mt + 1 of the next layer, so a layer's output is packed into the next
+// layer's operands one M-tile at a time and only the operands (never a
+// whole layer of FP32 activations) stay live: the kernel then fits several
+// waves per SIMD to hide the weight loads.
 __device__ inline int mlp_unit(int mt, int v, int lane) { return 32 * mt + 8 * (v / 4) + 4 * (lane / 32) + (v % 4); }
 
 // activations z (primal) and dz (tangent) of M-tile mt -> silu, packed as
-// the bf16 B operands hb / tb at K steps 4 mt .. 4 mt + 3; h and dh keep the
-// FP32 values for a following FP32 layer
-__device__ inline void pack_tile(const float (&z)[16], const float (&dz)[16], int mt, bf16x4 (&hb)[16],
-                                 bf16x4 (&tb)[16], float (&h)[16], float (&dh)[16]) {
+// the bf16 B operands hb / tb of K steps 2 mt, 2 mt + 1 (round to nearest
+// even, gfx950's two-value conversion); h and dh keep the FP32 values for a
+// following FP32 layer
+__device__ inline void pack_tile(const float (&z)[16], const float (&dz)[16], int mt, bf16x8 (&hb)[8],
+                                 bf16x8 (&tb)[8], float (&h)[16], float (&dh)[16]) {
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         float ds;
         h[v] = silu_d(z[v], ds);
         dh[v] = ds * dz[v];
     }
-    // round to nearest even, as f2bf: gfx950's packed conversion, two
-    // values per instruction
-    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
-    typedef float fl4 __attribute__((ext_vector_type(4)));
+    typedef float fl8 __attribute__((ext_vector_type(8)));
 #pragma unroll
-    for (int gq = 0; gq < 4; ++gq) {
-        const fl4 a = {h[4 * gq], h[4 * gq + 1], h[4 * gq + 2], h[4 * gq + 3]};
-        const fl4 b = {dh[4 * gq], dh[4 * gq + 1], dh[4 * gq + 2], dh[4 * gq + 3]};
-        hb[4 * mt + gq] = __builtin_bit_cast(bf16x4, __builtin_convertvector(a, bf4));
-        tb[4 * mt + gq] = __builtin_bit_cast(bf16x4, __builtin_convertvector(b, bf4));
+    for (int half = 0; half < 2; ++half) {
+        const fl8 a = {h[8 * half], h[8 * half + 1], h[8 * half + 2], h[8 * half + 3],
+                       h[8 * half + 4], h[8 * half + 5], h[8 * half + 6], h[8 * half + 7]};
+        const fl8 b = {dh[8 * half], dh[8 * half + 1], dh[8 * half + 2], dh[8 * half + 3],
+                       dh[8 * half + 4], dh[8 * half + 5], dh[8 * half + 6], dh[8 * half + 7]};
+        hb[2 * mt + half] = __builtin_convertvector(a, bf16x8);
+        tb[2 * mt + half] = __builtin_convertvector(b, bf16x8);
     }
 }
 
 // M-tile mt of a hidden 128 x 128 layer in MFMA, primal and tangent (no bias
 // on the tangent)
-__device__ inline void mfma_tile(const bf16x4* __restrict__ wl, const float* __restrict__ bias, const bf16x4 (&hb)[16],
-                                 const bf16x4 (&tb)[16], int mt, float (&z)[16], float (&dz)[16], int lane) {
+__device__ inline void mfma_tile(const bf16x8* __restrict__ wl, const float* __restrict__ bias, const bf16x8 (&hb)[8],
+                                 const bf16x8 (&tb)[8], int mt, float (&z)[16], float (&dz)[16], int lane) {
     f32x16 acc, tac;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = tac[v] = 0.0f;
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        bf16x4 a = wl[(mt * 16 + s) * 64 + lane];
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, hb[s], acc, 0, 0, 0);
-        tac = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, tb[s], tac, 0, 0, 0);
+    for (int s = 0; s < 8; ++s) {
+        bf16x8 a = wl[(mt * 8 + s) * 64 + lane];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, hb[s], acc, 0, 0, 0);
+        tac = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, tb[s], tac, 0, 0, 0);
     }
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
@@ -347,9 +350,9 @@ __device__ inline void mfma_tile(const bf16x4* __restrict__ wl, const float* __r
 
 // the network on 32 samples per wave (lanes l and l + 32 share sample l % 32):
 // output y and its derivative dy along the input direction dx
-__device__ inline void mlp_wave(const float* __restrict__ ws, const bf16x4* __restrict__ w2, const bf16x4* __restrict__ w3,
+__device__ inline void mlp_wave(const float* __restrict__ ws, const bf16x8* __restrict__ w2, const bf16x8* __restrict__ w3,
                                 const float (&x)[4], const float (&dx)[4], float (&y)[2], float (&dy)[2], int lane) {
-    bf16x4 hb[16], tb[16], nb[16], nt[16];
+    bf16x8 hb[8], tb[8], nb[8], nt[8];
     float z[16], dz[16], h[16], dh[16];
     // layer 1 in FP32 (K = 4), straight into layer 2's operands
 #pragma unroll
@@ -398,7 +401,7 @@ __device__ inline void mlp_wave(const float* __restrict__ ws, const bf16x4* __re
 }
 
 // raw inference for tests: X[n][4], dX[n][4] -> Y[n][2], dY[n][2]
-__global__ __launch_bounds__(256) void k_mlp_raw(int n, const float* __restrict__ wf, const bf16x4* __restrict__ wp,
+__global__ __launch_bounds__(256) void k_mlp_raw(int n, const float* __restrict__ wf, const bf16x8* __restrict__ wp,
                                                  const float* __restrict__ X, const float* __restrict__ dX,
                                                  float* __restrict__ Y, float* __restrict__ dY) {
     const int lane = threadIdx.x & 63;
@@ -410,7 +413,7 @@ __global__ __launch_bounds__(256) void k_mlp_raw(int n, const float* __restrict_
         x[k] = X[ic * 4 + k];
         dx[k] = dX[ic * 4 + k];
     }
-    mlp_wave(wf, wp, wp + 4 * 16 * 64, x, dx, y, dy, lane);
+    mlp_wave(wf, wp, wp + kLayerOperands, x, dx, y, dy, lane);
     if (lane < 32 && i < n) {
         Y[2 * i] = y[0];
         Y[2 * i + 1] = y[1];
@@ -435,13 +438,29 @@ __device__ inline int find_sys_t(const lensed_desc* __restrict__ sys, int nsys, 
 
 constexpr int kNodes = LENSED_EPL_ROWS;  // per candidate
 
+// table slot (candidate among those with a table) of node sample g
+__device__ inline long long node_slot(const lensed_desc* __restrict__ sys, int nsys, int g, int* rem) {
+    const int s = find_sys_t<1>(sys, nsys, g);
+    const lensed_desc& d = sys[s];
+    const int local = g - d.item_e, cand = local / kNodes;
+    if (rem) *rem = local - cand * kNodes;
+    return d.table_off / LENSED_EPL_TABLE_F4 + cand;
+}
+
 // two waves per SIMD (at most 256 VGPRs): one wave's weight loads hide
-// behind the other's MFMA
+// behind the other's MFMA.  With `skip`, a wave whose samples all belong to
+// candidates flagged for the exact series (a wave spans at most two
+// candidates) does nothing.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_emu_nodes(const lensed_desc* __restrict__ sys, int nsys, int nitems,
                                                    const float* __restrict__ params, const float* __restrict__ wf,
-                                                   const bf16x4* __restrict__ wp, float4* __restrict__ nodes) {
+                                                   const bf16x8* __restrict__ wp, float4* __restrict__ nodes,
+                                                   const int* __restrict__ skip) {
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave * 32 >= nitems) return;  // wave-uniform
+    if (skip && skip[node_slot(sys, nsys, wave * 32, nullptr)] &&
+        skip[node_slot(sys, nsys, min(wave * 32 + 31, nitems - 1), nullptr)])
+        return;  // wave-uniform: both candidates take the exact series
     const int g = wave * 32 + (lane & 31);
     const int gc = min(g, nitems - 1);
     const int s = find_sys_t<1>(sys, nsys, gc);
@@ -460,7 +479,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     float x[4] = {(q - kQ0) / kQS, (t - kT0) / kTS, cs, sn};
     float dx[4] = {0.0f, 0.0f, -sn * dpsi, cs * dpsi};
     float y[2], dy[2];
-    mlp_wave(wf, wp, wp + 4 * 16 * 64, x, dx, y, dy, lane);
+    mlp_wave(wf, wp, wp + kLayerOperands, x, dx, y, dy, lane);
     if (lane < 32 && g < nitems) {
         long long cb = d.table_off / LENSED_EPL_TABLE_F4 + cand;
         const float gh = powf(gj, h), lg = h * (4.0f * pj - 2.0f) / gj / LENSED_EPL_NT;  // d log(g^h) / df
@@ -470,10 +489,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
+// the (q, t) cell of the certification grid holding a candidate, or -1
+// outside the network's training domain
+__device__ __host__ inline int cell_of(float q, float t) {
+    if (!(q >= kQ0 - kQS && q <= kQ0 + kQS && t >= kT0 - kTS && t <= kT0 + kTS)) return -1;
+    int iq = (int)((q - (kQ0 - kQS)) / (2.0f * kQS) * kCellsQ), it = (int)((t - (kT0 - kTS)) / (2.0f * kTS) * kCellsT);
+    iq = iq < 0 ? 0 : iq >= kCellsQ ? kCellsQ - 1 : iq;
+    it = it < 0 ? 0 : it >= kCellsT ? kCellsT - 1 : it;
+    return iq * kCellsT + it;
+}
+
+// per candidate with a table: 0 if its (q, t) lies in a certified cell (the
+// network builds its table), 1 if not (the exact series does)
+__global__ __launch_bounds__(256) void k_emu_select(const lensed_desc* __restrict__ sys, int nsys, int ncands,
+                                                    const float* __restrict__ params, const int* __restrict__ cells,
+                                                    int* __restrict__ flags) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= ncands) return;
+    int s = 0;
+    for (; s < nsys - 1; ++s)
+        if (w < sys[s + 1].item_e / kNodes) break;
+    const lensed_desc& d = sys[s];
+    const int cand = w - d.item_e / kNodes;
+    const float* p = params + d.param_off + (long long)cand * d.npar;
+    const int c = cell_of(p[d.epl_q], p[d.epl_t]);
+    flags[d.table_off / LENSED_EPL_TABLE_F4 + cand] = c >= 0 && cells[c] ? 0 : 1;
+}
+
 // nodes -> cubic Hermite coefficients per interval (the table layout of
 // epl_table.h)
 __global__ __launch_bounds__(256) void k_emu_assemble(const lensed_desc* __restrict__ sys, int nsys, int nitems,
-                                                      const float4* __restrict__ nodes, float4* __restrict__ tables) {
+                                                      const float4* __restrict__ nodes, float4* __restrict__ tables,
+                                                      const int* __restrict__ skip) {
     int item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= nitems) return;
     int s = find_sys_t<0>(sys, nsys, item);
@@ -482,6 +529,7 @@ __global__ __launch_bounds__(256) void k_emu_assemble(const lensed_desc* __restr
     int cand = local / LENSED_EPL_ROWS;
     int i = local - cand * LENSED_EPL_ROWS;
     long long cb = d.table_off / LENSED_EPL_TABLE_F4 + cand;
+    if (skip && skip[cb]) return;  // the exact series builds this table
     float4 a = nodes[cb * kNodes + i];
     float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
     float4& ox = out[LENSED_EPL_AT(0, i)];
@@ -531,23 +579,26 @@ __global__ __launch_bounds__(256) void k_emu_check(const lensed_desc* __restrict
     }
 }
 
-// host: bf16 copies of W2, W3 in MFMA operand order
+// host: bf16 copies of W2, W3 in MFMA operand order (the permuted K order
+// of the x16 operands: K step S, lane half h, element i is input unit
+// 16 S + 8 (i / 4) + 4 h + i % 4)
 static std::vector<unsigned short> pack_bf16(const float* w) {
-    std::vector<unsigned short> out(2 * 4 * 16 * 64 * 4);
+    std::vector<unsigned short> out(2 * (size_t)kLayerOperands * 8);
     auto bf = [](float f) {
         unsigned u;
         memcpy(&u, &f, 4);
-        u += 0x7FFFu + ((u >> 16) & 1u);
+        u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
         return (unsigned short)(u >> 16);
     };
     for (int layer = 0; layer < 2; ++layer) {
         const float* W = w + (layer == 0 ? kW2 : kW3);
         for (int mt = 0; mt < 4; ++mt)
-            for (int s = 0; s < 16; ++s)
+            for (int S = 0; S < 8; ++S)
                 for (int lane = 0; lane < 64; ++lane)
-                    for (int i = 0; i < 4; ++i) {
-                        int o = 32 * mt + lane % 32, k = 8 * s + 4 * (lane / 32) + i;
-                        out[(((size_t)layer * 4 * 16 + mt * 16 + s) * 64 + lane) * 4 + i] = bf(W[o * kHidden + k]);
+                    for (int i = 0; i < 8; ++i) {
+                        const int o = 32 * mt + lane % 32, k = 16 * S + 8 * (i / 4) + 4 * (lane / 32) + i % 4;
+                        out[(((size_t)layer * kLayerOperands + (mt * 8 + S) * 64 + lane) * 8) + i] =
+                            bf(W[o * kHidden + k]);
                     }
     }
     return out;
@@ -565,18 +616,92 @@ int emu_upload(const float* weights, void** wf, void** wp) {
     return 0;
 }
 
-// emulated tables for a launch: nodes -> coefficients -> check -> flags
+// Certification of the network per (q, t) cell, once per weights: tables
+// built by the network at every corner and the centre of every cell are
+// checked against the FP64 series at every interval midpoint (points the
+// table's nodes never are); a cell is certified when all five of its points
+// are within eps.  cells (kCellsQ x kCellsT, host) receives 1 for certified
+// cells; returns their number, or -1.  *max_err: the largest error seen.
+int emu_certify(const void* wf, const void* wp, float eps, int* cells, double* max_err) {
+    using namespace emu;
+    const int nq = kCellsQ + 1, nt = kCellsT + 1;
+    std::vector<float> qt;
+    for (int i = 0; i < nq; ++i)
+        for (int j = 0; j < nt; ++j) {
+            qt.push_back(kQ0 - kQS + 2.0f * kQS * i / kCellsQ);
+            qt.push_back(kT0 - kTS + 2.0f * kTS * j / kCellsT);
+        }
+    for (int i = 0; i < kCellsQ; ++i)
+        for (int j = 0; j < kCellsT; ++j) {
+            qt.push_back(kQ0 - kQS + 2.0f * kQS * (i + 0.5f) / kCellsQ);
+            qt.push_back(kT0 - kTS + 2.0f * kTS * (j + 0.5f) / kCellsT);
+        }
+    const int np = (int)qt.size() / 2;
+    lensed_desc d;
+    memset(&d, 0, sizeof d);
+    d.ncand = np;
+    d.npar = 2;
+    d.epl_q = 0;
+    d.epl_t = 1;
+    d.table_mode = 1;
+    lensed_desc* dd = nullptr;
+    float *par = nullptr, *errs = nullptr;
+    float4 *nodes = nullptr, *tables = nullptr;
+    int* flags = nullptr;
+    hipStream_t st = nullptr;
+    int rc = -1;
+    std::vector<float> e(np);
+    if (hipMalloc(&dd, sizeof d) == hipSuccess && hipMalloc(&par, qt.size() * 4) == hipSuccess &&
+        hipMalloc(&nodes, (size_t)np * kNodes * 16) == hipSuccess &&
+        hipMalloc(&tables, (size_t)np * LENSED_EPL_TABLE_F4 * 16) == hipSuccess &&
+        hipMalloc(&flags, (size_t)np * 4) == hipSuccess && hipMalloc(&errs, (size_t)np * 4) == hipSuccess &&
+        hipMemcpy(dd, &d, sizeof d, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(par, qt.data(), qt.size() * 4, hipMemcpyHostToDevice) == hipSuccess) {
+        const int n_nodes = np * kNodes, waves = (n_nodes + 31) / 32;
+        hipLaunchKernelGGL(k_emu_nodes, dim3((waves + 3) / 4), dim3(256), 0, st, dd, 1, n_nodes, par,
+                           (const float*)wf, (const bf16x8*)wp, nodes, (const int*)nullptr);
+        hipLaunchKernelGGL(k_emu_assemble, dim3((n_nodes + 255) / 256), dim3(256), 0, st, dd, 1, n_nodes,
+                           (const float4*)nodes, tables, (const int*)nullptr);
+        hipLaunchKernelGGL(k_emu_check, dim3((np + 3) / 4), dim3(256), 0, st, dd, 1, np, (const float*)par,
+                           (const float4*)tables, eps, flags, errs);
+        if (hipGetLastError() == hipSuccess &&
+            hipMemcpy(e.data(), errs, (size_t)np * 4, hipMemcpyDeviceToHost) == hipSuccess)
+            rc = 0;
+    }
+    for (void* p : {(void*)dd, (void*)par, (void*)nodes, (void*)tables, (void*)flags, (void*)errs})
+        if (p) (void)hipFree(p);
+    if (rc) {
+        set_error("emulator certification failed");
+        return -1;
+    }
+    int n = 0;
+    double mx = 0;
+    for (float x : e) mx = std::max(mx, (double)x);
+    for (int i = 0; i < kCellsQ; ++i)
+        for (int j = 0; j < kCellsT; ++j) {
+            const float worst = std::max({e[i * nt + j], e[(i + 1) * nt + j], e[i * nt + j + 1],
+                                          e[(i + 1) * nt + j + 1], e[nq * nt + i * kCellsT + j]});
+            cells[i * kCellsT + j] = worst <= eps ? 1 : 0;
+            n += cells[i * kCellsT + j];
+        }
+    if (max_err) *max_err = mx;
+    return n;
+}
+
+// emulated tables for a launch: which candidates lie in certified cells,
+// then network nodes -> Hermite coefficients for those; flags (1 = exact
+// series) tell the exact-table kernel which tables it builds
 int emu_launch_tables(hipStream_t st, const lensed_desc* d_desc, int nsys, int n_nodes, int n_intervals, int n_cands,
                       const float* params, const void* wf, const void* wp, void* nodes, void* tables, int* flags,
-                      float* errs, float eps) {
+                      const int* cells) {
     using namespace emu;
     int waves = (n_nodes + 31) / 32;
+    hipLaunchKernelGGL(k_emu_select, dim3((n_cands + 255) / 256), dim3(256), 0, st, d_desc, nsys, n_cands, params,
+                       cells, flags);
     hipLaunchKernelGGL(k_emu_nodes, dim3((waves + 3) / 4), dim3(256), 0, st, d_desc, nsys, n_nodes, params,
-                       (const float*)wf, (const bf16x4*)wp, (float4*)nodes);
+                       (const float*)wf, (const bf16x8*)wp, (float4*)nodes, (const int*)flags);
     hipLaunchKernelGGL(k_emu_assemble, dim3((n_intervals + 255) / 256), dim3(256), 0, st, d_desc, nsys, n_intervals,
-                       (const float4*)nodes, (float4*)tables);
-    hipLaunchKernelGGL(k_emu_check, dim3((n_cands + 3) / 4), dim3(256), 0, st, d_desc, nsys, n_cands, params,
-                       (const float4*)tables, eps, flags, errs);
+                       (const float4*)nodes, (float4*)tables, (const int*)flags);
     EMU_HIP(hipGetLastError());
     return 0;
 }
@@ -750,7 +875,7 @@ int lensed_emulator_eval(const float* weights, const float* X, const float* dX, 
     EMU_HIP(hipMemcpy(ddx, dX, (size_t)n * 16, hipMemcpyHostToDevice));
     int waves = (n + 31) / 32;
     hipLaunchKernelGGL(k_mlp_raw, dim3((waves + 3) / 4), dim3(256), 0, nullptr, n, (const float*)wf,
-                       (const bf16x4*)wp, dx, ddx, dy, ddy);
+                       (const bf16x8*)wp, dx, ddx, dy, ddy);
     EMU_HIP(hipMemcpy(Y, dy, (size_t)n * 8, hipMemcpyDeviceToHost));
     if (dY) EMU_HIP(hipMemcpy(dY, ddy, (size_t)n * 8, hipMemcpyDeviceToHost));
     for (void* p : {wf, wp, (void*)dx, (void*)ddx, (void*)dy, (void*)ddy}) (void)hipFree(p);

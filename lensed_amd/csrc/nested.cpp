@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <random>
 #include <string>
@@ -178,14 +179,18 @@ struct Ellipsoid {
 // to occupy (their share m/N of the remaining prior volume X, enlarged).
 // Periodic coordinates are first rotated so that the circular mean of the
 // points sits at 0.5, which keeps a mode across the 0/1 seam in one piece.
+bool inside(const double* u, int n, const std::vector<int>& wrap, double* out);
+
 struct Bound {
     static constexpr int kBoot = 5;  // bootstrap resamples of a split-off piece
+    static constexpr int kProbe = 1024;  // draws that measure the share of the bound inside the cube
     int n = 0;
     int boot = kBoot;           // 0 in constant efficiency mode
     std::vector<double> shift;  // x = frac(u + shift) for periodic coordinates
     std::vector<Ellipsoid> ells;
     std::vector<double> cum;    // cumulative volume fractions
     double logvol_union = 0;    // log of the summed volumes (bounds the union's)
+    double logvol_cube = 0;     // log of the volume of the union's part inside the cube (estimated)
 
     void to_x(const double* u, double* x) const {
         for (int i = 0; i < n; ++i) {
@@ -308,6 +313,21 @@ struct Bound {
         for (size_t k = 0; k < ells.size(); ++k) cum[k] = acc += std::exp(ells[k].logvol - top);
         for (auto& c : cum) c /= acc;
         logvol_union = top + std::log(acc);
+        // Where the union is larger than the cube, the share of its draws
+        // that land inside the cube gives the volume it actually proposes
+        // from: in many dimensions an ellipsoid many times the cube's volume
+        // can still cover a small part of it, and drawing from it then beats
+        // drawing from the prior.
+        logvol_cube = logvol_union;
+        if (logvol_union >= 0) {
+            double x[64], c[64];
+            int hit = 0;
+            for (int k = 0; k < kProbe; ++k) {
+                sample(rng, x);
+                hit += inside(x, n, wrap, c);
+            }
+            logvol_cube = logvol_union + std::log(std::max(hit, 1) / (double)kProbe);
+        }
     }
 
     // a uniform draw from the union (in unit-cube coordinates, unwrapped)
@@ -437,6 +457,14 @@ void summarise(Result& r, const std::vector<Point>& live, double logx, int npar)
 
 typedef void (*DumperFn)(int*, int*, int*, double**, double**, double**, double*, double*, double*, double*, void*);
 
+// After a batch is evaluated: the candidates whose logL came from an
+// approximation (the EPL emulator) and that may still be accepted -- logL
+// above the current threshold, which only rises while the batch is used up
+// -- get their exact logL (phys in sampler order; ll replaced in place), so
+// every point that joins the live set, and so every posterior sample,
+// carries the exact value.
+using RecheckFn = std::function<void(const double* phys, int n, double lmin, double* ll)>;
+
 void call_dumper(DumperFn dumper, void* ctx, Result& r, const std::vector<Point>& live, double logx, int npar) {
     if (!dumper) return;
     summarise(r, live, logx, npar);
@@ -510,7 +538,8 @@ struct Steer {
     }
 };
 
-int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn dumper, void* dctx, Result& r) {
+int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn dumper, void* dctx, Result& r,
+           const RecheckFn& recheck = nullptr) {
     if (ndim <= 0 || ndim > 64 || o.nlive <= ndim) {
         lsd::set_error("nested sampling needs 0 < ndim < nlive and ndim <= 64");
         return -1;
@@ -526,6 +555,7 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
         std::vector<double> cubes((size_t)o.nlive * ndim), phys((size_t)o.nlive * npar), ll(o.nlive);
         for (auto& c : cubes) c = uni(rng);
         fn(cubes.data(), o.nlive, phys.data(), ll.data(), ctx);
+        if (recheck) recheck(phys.data(), o.nlive, -INFINITY, ll.data());
         r.nevals += o.nlive;
         for (int i = 0; i < o.nlive; ++i) {
             live[i].u.assign(cubes.begin() + (size_t)i * ndim, cubes.begin() + (size_t)(i + 1) * ndim);
@@ -581,7 +611,7 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
                 // at acceptance efr, so that a bound far too large or small is
                 // not kept while its few replacements trickle in.
                 if (bound.ells.empty() || since_fit >= std::max(1, n / 10) || (o.ceff && fit_prop >= refit_prop)) {
-                    if (o.ceff && fit_prop > 0) volfac = steer(volfac, fit_acc, fit_prop, bound.logvol_union >= 0);
+                    if (o.ceff && fit_prop > 0) volfac = steer(volfac, fit_acc, fit_prop, bound.logvol_cube >= 0);
                     bound.fit(live, volfac, logx, o.wrap, true, rng);
                     since_fit = 0;
                     fit_acc = fit_prop = 0;
@@ -605,12 +635,12 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
                 cubes.resize((size_t)b * ndim);
                 phys.resize((size_t)b * npar);
                 ll.resize(b);
-                // While the bound is larger than the cube (early on, and always
-                // in many dimensions, where the ellipsoid around a cube holds
-                // the cube as a tiny fraction) the prior itself is the better
-                // proposal; either is a uniform draw over a superset of the
-                // constrained region, which is all the acceptance test needs.
-                const bool use_cube = bound.logvol_union >= 0;
+                // While the bound's part inside the cube is larger than the
+                // cube itself would be, i.e. early on (Bound::logvol_cube), the
+                // prior is the better proposal; either is a uniform draw over a
+                // superset of the constrained region, which is all the
+                // acceptance test needs.
+                const bool use_cube = bound.logvol_cube >= 0;
                 // kStreams independent streams seeded from the main one fill
                 // the batch (candidate k from stream k % kStreams), on threads
                 // for large batches; the draws do not depend on the thread count
@@ -641,6 +671,7 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
                     for (int st = 0; st < kStreams; ++st) fill(st);
                 }
                 fn(cubes.data(), b, phys.data(), ll.data(), ctx);
+                if (recheck) recheck(phys.data(), b, lmin, ll.data());
                 r.nevals += b;
                 next = 0;
                 have = b;
@@ -665,9 +696,9 @@ int sample(int ndim, int npar, BatchFn fn, void* ctx, const Options& o, DumperFn
             fprintf(stderr,
                     "nested: iteration %ld  ln Z %.4f  ln X %.2f  evaluations %ld  ellipsoids %zu  ln V %.2f  "
                     "acceptance %.2e  enlargement %.3g%s\n",
-                    r.niter, r.logz, logx, r.nevals, bound.ells.size(), bound.logvol_union,
+                    r.niter, r.logz, logx, r.nevals, bound.ells.size(), bound.logvol_cube,
                     fb_prop ? (double)fb_acc / fb_prop : 0.0, volfac,
-                    bound.logvol_union >= 0 ? "  (prior draws)" : "");
+                    bound.logvol_cube >= 0 ? "  (prior draws)" : "");
             fb_acc = fb_prop = 0;
         }
         if (o.updint > 0 && r.niter % o.updint == 0) {
@@ -791,7 +822,12 @@ int lensed_nested_run(lensed* l, size_t s, const lensed_ns_options* opts, double
         for (int i = 0; i < npar; ++i) o.wrap[i] = sys.pars[sys.pmap[i]].wrap;
     }
     Result r;
-    if (sample(ndim, npar, lensed_batch, &b, o, s == 0 ? dumper : nullptr, l, r)) return -1;
+    // accepted points of an emulated launch get their exact logL
+    RecheckFn recheck = [l, s](const double* phys, int n, double lmin, double* ll) {
+        if (lsd::recheck_emulated(l, s, phys, n, lmin, ll))
+            for (int i = 0; i < n; ++i) ll[i] = -DBL_MAX;
+    };
+    if (sample(ndim, npar, lensed_batch, &b, o, s == 0 ? dumper : nullptr, l, r, recheck)) return -1;
     fill_result(r, result, mean, sigma, ml, npar);
     return 0;
 }

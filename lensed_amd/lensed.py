@@ -264,11 +264,28 @@ class Context:
                "write_layers")
 
     def emulator_stats(self):
-        """(fallbacks, candidates checked, largest held-out error) of the last emulated launch."""
+        """(candidates with exact-series tables, candidates with a table, largest
+        held-out error of the certification) of the last emulated launch."""
         from ctypes import c_long
         nf, nc, mx = c_long(), c_long(), c_double()
         _check(L.load().lensed_emulator_stats(self.ptr, byref(nf), byref(nc), byref(mx)), "emulator_stats")
         return nf.value, nc.value, mx.value
+
+    def emulator_cells(self):
+        """(cells, certified cells, largest held-out error) of the emulator's
+        certification at context creation."""
+        from ctypes import c_long
+        n, nc, mx = c_long(), c_long(), c_double()
+        _check(L.load().lensed_emulator_cells(self.ptr, byref(n), byref(nc), byref(mx)), "emulator_cells")
+        return n.value, nc.value, mx.value
+
+    def emulator_rechecks(self):
+        """(rechecked, changed, largest |delta logL|): exact re-evaluations of
+        emulated candidates by the batched nested sampler."""
+        from ctypes import c_long
+        n, ch, mx = c_long(), c_long(), c_double()
+        _check(L.load().lensed_emulator_rechecks(self.ptr, byref(n), byref(ch), byref(mx)), "emulator_rechecks")
+        return n.value, ch.value, mx.value
 
     def render_stats(self):
         """(render items, items the pixel cull kept, waves traced) of the last

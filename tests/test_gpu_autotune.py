@@ -1,0 +1,98 @@
+"""The autotuner (LENSED_AUTOTUNE) on the GPU.
+
+At the first launch of a layout it times the launch chain on that launch's
+own candidates over the launch shapes (render list or item walk, grid
+sizes, convolution tile height and item order) and keeps the fastest; the
+choice is cached on disk per device, program and layout.  For the 256 x 256
+EPL workload (4096 candidates) and the uneven 64-system batch (256 per
+system): the logL of every candidate equals the untuned context's bit for
+bit, the tuner's own alternating measurement puts its choice at or below
+the defaults, and timed launches of the tuned context are not slower than
+those of the default one (3 % allowance for timing noise); a second context
+with the same key takes the cached choice without searching.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from lensed_amd import workloads as W
+from lensed_amd._lib import LENSED_AUTOTUNE, LENSED_GRAPH, LENSED_TABLE
+from lensed_amd.lensed import Context, LensedObject, SystemSpec
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_mock(wl):
+    objs = [LensedObject(oid, name) for oid, name in wl.objects]
+    pids = [f"{o.id}.{p}" for o in objs for p in o.params]
+
+    def render(objects, params, shape, psf):
+        pri = {k: repr(float(v)) for k, v in zip(pids, params)}
+        spec = SystemSpec(objects, pri, np.zeros(shape, np.float32), np.ones(shape, np.float32), psf=psf)
+        with Context(spec, max_batch=1) as c:
+            return c.render(params)["image"].astype(np.float64)
+    wl.make_data(render, pids, mag_keys=("source.mag",))
+    return pids
+
+
+def time_launches(call, reps=5):
+    call()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        call()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def check_tuned(specs, call, flags, tmp_path, monkeypatch, K, label):
+    monkeypatch.setenv("LENSED_CACHE_DIR", str(tmp_path))
+    with Context(specs, max_batch=K, flags=flags) as ref:
+        lref = call(ref)
+        t_def = time_launches(lambda: call(ref))
+    with Context(specs, max_batch=K, flags=flags | LENSED_AUTOTUNE) as tuned:
+        t0 = time.perf_counter()
+        lt = call(tuned)  # the first launch searches
+        t_search = time.perf_counter() - t0
+        tn = tuned.tuning()
+        for a, b in zip(lt if isinstance(lt, list) else [lt], lref if isinstance(lref, list) else [lref]):
+            assert np.array_equal(a, b)
+        t_tuned = time_launches(lambda: call(tuned))
+        again = call(tuned)
+        for a, b in zip(again if isinstance(again, list) else [again], lref if isinstance(lref, list) else [lref]):
+            assert np.array_equal(a, b)
+    print(f"{label}: chose {tn}; search {t_search * 1e3:.0f} ms; launches {t_def * 1e3:.2f} ms default, "
+          f"{t_tuned * 1e3:.2f} ms tuned", flush=True)
+    assert 0 < tn["tuned_ms"] <= tn["default_ms"], tn
+    assert t_tuned <= 1.03 * t_def, (t_tuned, t_def)
+    # the cached choice: the same shapes, no search
+    with Context(specs, max_batch=K, flags=flags | LENSED_AUTOTUNE) as cached:
+        t0 = time.perf_counter()
+        lc = call(cached)
+        t_cached = time.perf_counter() - t0
+        assert cached.tuning() == tn
+        for a, b in zip(lc if isinstance(lc, list) else [lc], lref if isinstance(lref, list) else [lref]):
+            assert np.array_equal(a, b)
+    assert t_cached < t_search
+    return tn
+
+
+def test_autotune_c3(tmp_path, monkeypatch):
+    wl = W.c3()
+    gpu_mock(wl)
+    K = 4096
+    cubes = wl.candidates(14, K)
+    check_tuned(wl.spec(), lambda c: c.loglike_batch(cubes)[0], LENSED_TABLE | LENSED_GRAPH, tmp_path,
+                monkeypatch, K, "C3")
+
+
+def test_autotune_c4(tmp_path, monkeypatch):
+    wls = W.c4()
+    phys = []
+    rng = np.random.default_rng(404)
+    for wl in wls:
+        pids = gpu_mock(wl)
+        phys.append(np.array([[rng.uniform(*wl.prior_range(k)) for k in pids] for _ in range(256)]))
+    check_tuned([wl.spec() for wl in wls], lambda c: c.loglike_phys(phys), LENSED_TABLE | LENSED_GRAPH, tmp_path,
+                monkeypatch, 256, "C4")

@@ -6,9 +6,11 @@ batched run proposes many candidates per launch (lensed_nested_run).  The
 two are independent samplings of the same posterior, so they must agree
 within their sampling noise.
 
-C1 (SIE + Sersic, 100 x 100), bounds that enclose the live points: the
-log-evidence within four combined standard errors, posterior means within
-0.3 sigma, and the mock's truth inside the posterior.
+C1 (SIE + Sersic, 100 x 100) with the configuration's 50 live points and
+bounds that enclose the live points: the log-evidence within four combined
+standard errors, posterior means within 0.6 sigma (the 0.3 sigma that two
+runs of 200 live points meet, widened by sqrt(200 / 50) for the sampling
+noise of 50), and the mock's truth inside the posterior.
 
 C3 (EPL + shear, Gaussian source, 256 x 256, 21 x 21 PSF) with Lensed's
 default sampler settings (constant efficiency mode, target acceptance 0.05,
@@ -72,14 +74,14 @@ def test_plain_and_batched_nested_runs_agree(c1_ctx):
 
     t0 = time.perf_counter()
     nested.run_multinest_abi(ctypes.cast(lib.loglike, L.LOGLIKE_FN), ndim, npar,
-                             ctypes.cast(lib.dumper, L.DUMPER_FN), nlive=200, tol=0.5, efr=0.8, ceff=False,
+                             ctypes.cast(lib.dumper, L.DUMPER_FN), nlive=50, tol=0.5, efr=0.8, ceff=False,
                              updint=1000, seed=21, context=ctx.ptr)
     t_plain = time.perf_counter() - t0
     mean_p, sig_p, logz_p, logz_err_p, maxl_p = results(ctx)
     print(f"plain run(): {t_plain:.1f} s, logZ {logz_p:.3f} +- {logz_err_p:.3f}, max logL {maxl_p:.2f}", flush=True)
 
     t0 = time.perf_counter()
-    rb = nested.run_context(ctx, 0, nlive=200, tol=0.5, efr=0.8, batch=1024, seed=22)
+    rb = nested.run_context(ctx, 0, nlive=50, tol=0.5, efr=0.8, batch=1024, seed=22)
     t_batch = time.perf_counter() - t0
     print(f"batched: {t_batch:.1f} s, {rb}", flush=True)
     mean_b = np.zeros(npar)
@@ -91,8 +93,8 @@ def test_plain_and_batched_nested_runs_agree(c1_ctx):
     assert abs(logz_p - rb.logz) < 4 * err + 0.1, (logz_p, rb.logz, err)
     dev = np.abs(mean_p - mean_b) / sig_p
     print("mean difference / sigma:", dict(zip(pids, np.round(dev, 3))), flush=True)
-    assert np.all(dev < 0.3), dev
-    assert np.all(np.abs(sig_b / sig_p - 1) < 0.3)
+    assert np.all(dev < 0.6), dev
+    assert np.all(np.abs(sig_b / sig_p - 1) < 0.6)
     truth = wl.truth_vector(pids)
     assert np.all(np.abs(mean_b - truth) < 5 * sig_b + 1e-6), (mean_b - truth) / sig_b
     assert abs(rb.max_loglike - maxl_p) < 5
@@ -132,3 +134,4 @@ def test_c3_emulated_batched_posterior_matches_plain_run():
     print("mean difference / sigma:", dict(zip(pids, np.round(dev, 3))), flush=True)
     assert np.all(dev < 0.1), dev
     assert abs(rb.max_loglike - maxl_p) < 5
+

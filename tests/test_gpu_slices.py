@@ -4,10 +4,12 @@ image-plane priors, every quadrature rule, the 200 x 200 batched launch of
 read, the render's walk over culled items, and the launch-shape knobs.
 
 Bars as in test_gpu_parity: model images within 1e-4 relative of the FP64
-oracle where the model exceeds 1e-3 of its peak; |delta logL| <= 1e-2 near
-the truth and 1e-2 + 1e-5 |logL| for arbitrary prior draws (FP32 model
-rounding at |logL| up to 1e7); results under different launch shapes and
-culling choices equal bit for bit.
+oracle where the model exceeds 1e-3 of its peak (for prior draws, or twice
+the FP32 restatement's own error where that is larger: a compact Sersic
+cusp at pixel scale); |delta logL| <= 1e-2 near the truth and 1e-2 + 1e-5
+|logL| for arbitrary prior draws (FP32 model rounding at |logL| up to 1e7);
+results under different launch shapes and culling choices equal bit for
+bit.
 """
 import os
 
@@ -144,9 +146,14 @@ def test_c2_batched_launch_against_fp64_oracle(c2):
             worst = max(worst, abs(lnew[i] - lref) / (1 + abs(lref)))
             assert abs(lnew[i] - lref) <= 1e-2 + 1e-5 * abs(lref), (i, lnew[i], lref)
             if j < 4:
+                # a prior draw can put a compact Sersic cusp at pixel scale,
+                # where rounding the ray positions to FP32 alone moves the
+                # model by more than 1e-4: the bar is then the FP32
+                # restatement's own error (the reference's precision), twice
                 planes = ctx.render(phys[i])
                 rel, _ = rel_err(planes["image"], model)
-                assert rel <= 1e-4, (i, rel)
+                rel32, _ = rel_err(oracle_model(wl.objects, phys[i], wl.shape, wl.psf, np.float32), model)
+                assert rel <= max(1e-4, 2 * rel32), (i, rel, rel32)
         model = oracle_model(wl.objects, truth, wl.shape, wl.psf)
         lref = -0.5 * O.chi2_d(wl.image, wl.weight, model)
         assert abs(lt - lref) <= 1e-2, (lt, lref)
