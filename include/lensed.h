@@ -265,6 +265,15 @@ typedef struct {
     double tuned_ms;      /* with the knobs it chose */
 } lensed_tuning;
 
+/* Candidates per launch, tuned (the batch shape): lensed_loglike_batch() on
+ * system s timed over K = 256, 512, ... up to max_batch (seeded unit-cube
+ * draws, 3 launches each after one warm-up; launch shapes autotuned first
+ * under LENSED_AUTOTUNE); *best_k = the smallest K whose logL evals/s are
+ * within 5 % of the best.  ks and rates (may be NULL, room for 32) receive
+ * each K tried and its evals/s, *nk their number.  No counterpart in the
+ * reference. */
+int lensed_autotune_batch(lensed* l, size_t s, size_t* best_k, size_t* ks, double* rates, int* nk);
+
 /* knobs for launches from now on (layouts already built are rebuilt);
  * NULL restores the defaults.  Returns 0, or -1 for an invalid value. */
 int lensed_set_tuning(lensed* l, const lensed_tuning* t);
@@ -336,7 +345,8 @@ typedef struct {
     double tol;      /* stop when the live points can raise ln Z by < tol (0.1) */
     double efr;      /* target acceptance if ceff, else shrinking factor (0.8) */
     int ceff;        /* constant efficiency mode */
-    int batch;       /* candidates proposed per likelihood call */
+    int batch;       /* candidates proposed per likelihood call; in lensed_nested_run() -1 takes
+                        lensed_autotune_batch()'s choice */
     int seed;        /* random seed */
     int updint;      /* iterations between dumper calls (0 = only at the end) */
     int maxiter;     /* iteration limit (0 = none) */

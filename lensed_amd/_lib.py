@@ -129,6 +129,8 @@ SIGNATURES = {
     "lensed_emulator_rechecks": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
     "lensed_render_stats": (c_int, [c_void_p, POINTER(c_long), POINTER(c_long), POINTER(c_long), POINTER(c_double)]),
     "lensed_set_tuning": (c_int, [c_void_p, POINTER(Tuning)]),
+    "lensed_autotune_batch": (c_int, [c_void_p, c_size_t, POINTER(c_size_t), POINTER(c_size_t), POINTER(c_double),
+                                      POINTER(c_int)]),
     "lensed_get_tuning": (c_int, [c_void_p, POINTER(Tuning)]),
     "lensed_fits_read": (c_int, [c_char_p, POINTER(c_size_t), POINTER(c_size_t), POINTER(POINTER(c_float)),
                                  POINTER(c_double)]),

@@ -816,6 +816,11 @@ int lensed_nested_run(lensed* l, size_t s, const lensed_ns_options* opts, double
     int ndim = (int)lensed_ndims(l, s), npar = (int)lensed_npars(l, s);
     BatchCtx b{l, s, ndim};
     Options o = to_options(opts, npar);
+    if (opts && opts->batch < 0) {  // the batch shape from the autotuner
+        size_t k = 0;
+        if (lensed_autotune_batch(l, s, &k, nullptr, nullptr, nullptr)) return -1;
+        o.batch = (int)k;
+    }
     if (!opts || !opts->wrap) {  // the parameters' own periodic flags (sampler order), as Lensed passes them
         const auto& sys = l->sys[s];
         o.wrap.resize(npar);

@@ -309,6 +309,14 @@ class Context:
         _check(L.load().lensed_get_tuning(self.ptr, byref(t)), "get_tuning")
         return {name: getattr(t, name) for name, _ in L.Tuning._fields_}
 
+    def autotune_batch(self, s=0):
+        """(best candidates per launch, {K: logL evals/s}) from timing
+        lensed_loglike_batch() over K = 256 .. max_batch."""
+        best = c_size_t()
+        ks, rates, nk = (c_size_t * 32)(), (c_double * 32)(), c_int()
+        _check(L.load().lensed_autotune_batch(self.ptr, s, byref(best), ks, rates, byref(nk)), "autotune_batch")
+        return best.value, {ks[i]: rates[i] for i in range(nk.value)}
+
     def profile(self):
         """Accumulated stage times (ms) with the reference's stage names."""
         ms = (c_double * 8)()

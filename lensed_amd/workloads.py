@@ -146,13 +146,16 @@ def c5():
              "bulge.q": 0.8, "bulge.pa": 10.0,
              "disk.x": 258.0, "disk.y": 255.0, "disk.r": 9.0, "disk.mag": 0.5, "disk.n": 1.0,
              "disk.q": 0.5, "disk.pa": 120.0, "sky.bg": 0.005, "sky.dx": 0.0, "sky.dy": 0.0}
+    # position angles periodic, as Lensed's examples declare them
+    # (examples/full_mock_psf.ini:42,49)
     priors = {"lens.x": "unif 254 259", "lens.y": "unif 254 259", "lens.r": "unif 35 45",
-              "lens.t": "unif 0.5 1.5", "lens.q": "unif 0.4 1", "lens.pa": "unif 0 180",
+              "lens.t": "unif 0.5 1.5", "lens.q": "unif 0.4 1", "lens.pa": "wrap unif 0 180",
               "lens.g1": "unif -0.1 0.1", "lens.g2": "unif -0.1 0.1",
               "bulge.x": "unif 254 262", "bulge.y": "unif 251 259", "bulge.r": "unif 2 8", "bulge.mag": "unif -5 5",
-              "bulge.n": "unif 1 6", "bulge.q": "unif 0.4 1", "bulge.pa": "unif 0 180",
+              "bulge.n": "unif 1 6", "bulge.q": "unif 0.4 1", "bulge.pa": "wrap unif 0 180",
               "disk.x": "unif 254 262", "disk.y": "unif 251 259", "disk.r": "unif 4 16", "disk.mag": "unif -5 5",
-              "disk.n": "unif 0.5 2", "disk.q": "unif 0.2 1", "disk.pa": "unif 0 180", "sky.bg": "unif -0.05 0.05"}
+              "disk.n": "unif 0.5 2", "disk.q": "unif 0.2 1", "disk.pa": "wrap unif 0 180",
+              "sky.bg": "unif -0.05 0.05"}
     return Workload("C5", objects, truth, priors, (512, 512), 0.02, psf=moffat_psf(31, 3.0, 3.0), seed=5,
                     cand_seed=105, ncand=2048)
 

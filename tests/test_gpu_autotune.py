@@ -96,3 +96,21 @@ def test_autotune_c4(tmp_path, monkeypatch):
         phys.append(np.array([[rng.uniform(*wl.prior_range(k)) for k in pids] for _ in range(256)]))
     check_tuned([wl.spec() for wl in wls], lambda c: c.loglike_phys(phys), LENSED_TABLE | LENSED_GRAPH, tmp_path,
                 monkeypatch, 256, "C4")
+
+
+def test_autotune_batch_and_nested_run(tmp_path, monkeypatch):
+    """The batch shape: lensed_autotune_batch() times whole launches over
+    K = 256 .. max_batch and picks the smallest K within 5 % of the best
+    rate; lensed_nested_run() with batch = -1 runs with that choice."""
+    from lensed_amd import nested
+    monkeypatch.setenv("LENSED_CACHE_DIR", str(tmp_path))
+    wl = W.c3()
+    gpu_mock(wl)
+    with Context(wl.spec(), max_batch=4096, flags=LENSED_TABLE | LENSED_GRAPH | LENSED_AUTOTUNE) as ctx:
+        best, rates = ctx.autotune_batch()
+        print(f"C3 batch shapes (logL evals/s): {rates}; chosen {best}", flush=True)
+        assert sorted(rates) == [256, 512, 1024, 2048, 4096]
+        assert best in rates and rates[best] >= 0.95 * max(rates.values())
+        assert all(best <= k for k, r in rates.items() if r >= 0.95 * max(rates.values()))
+        rb = nested.run_context(ctx, 0, nlive=100, tol=0.5, efr=0.8, batch=-1, seed=71)
+    assert np.isfinite(rb.logz) and rb.nevals > 0
