@@ -44,11 +44,12 @@ class Workload:
     """Objects, truth, priors and data of one system, plus candidate draws."""
 
     def __init__(self, name, objects, truth, priors, shape, sigma, psf=None, seed=1, cand_seed=101,
-                 ncand=1, fixed=()):
+                 ncand=1, fixed=(), mag_window=None):
         self.name, self.objects, self.truth, self.priors = name, objects, dict(truth), dict(priors)
         self.shape, self.sigma, self.psf = shape, sigma, psf
         self.seed, self.cand_seed, self.ncand = seed, cand_seed, ncand
         self.fixed = set(fixed)
+        self.mag_window = mag_window
         self.image = self.weight = None
 
     def truth_vector(self, param_ids):
@@ -56,7 +57,8 @@ class Workload:
 
     def make_data(self, render, param_ids, peak_sigmas=10.0, mag_keys=()):
         """Scale the listed magnitudes so the brightest pixel is peak_sigmas
-        noise sigmas, then add seeded noise."""
+        noise sigmas, then add seeded noise.  With a magnitude window w, the
+        listed magnitudes' priors become uniform over the truth +- w."""
         p = self.truth_vector(param_ids)
         model = render(self.objects, p, self.shape, self.psf)
         if mag_keys:
@@ -64,6 +66,9 @@ class Workload:
             dm = -2.5 * math.log10(peak_sigmas * self.sigma / peak)
             for k in mag_keys:
                 self.truth[k] += dm
+                if self.mag_window:
+                    t, w = self.truth[k], self.mag_window
+                    self.priors[k] = f"unif {t - w:.6f} {t + w:.6f}"
             p = self.truth_vector(param_ids)
             model = render(self.objects, p, self.shape, self.psf)
         rng = np.random.default_rng(self.seed)
@@ -138,7 +143,18 @@ def c3():
 
 
 def c5():
-    """EPL + shear, two Sersic components and sky, 512 x 512, 31x31 Moffat PSF."""
+    """EPL + shear, two Sersic components and sky, 512 x 512, 31x31 Moffat PSF.
+
+    The priors are those of a follow-up fit: a few pixels around the lens and
+    source centres, sizes, indices and axis ratios over a factor of about
+    two, magnitudes within 1.5 of the truth (set by make_data), position
+    angles periodic over the whole half turn, as Lensed's examples declare
+    them (examples/full_mock_psf.ini:42,49).  With magnitudes free over ten
+    and positions over eight pixels, the bulk of the prior is a plateau of
+    faint or misplaced sources (logL within ~1700 of the best fit, the sky
+    alone fitted): after 6000 iterations of 400 live points the live set
+    still spanned nearly the whole cube in 21 of 23 dimensions and the
+    bounding ellipsoid accepted 3e-4 of its draws."""
     objects = [("lens", "epl_plus_shear"), ("bulge", "sersic"), ("disk", "sersic"), ("sky", "sky")]
     truth = {"lens.x": 256.5, "lens.y": 256.5, "lens.r": 40.0, "lens.t": 0.9, "lens.q": 0.75, "lens.pa": 70.0,
              "lens.g1": -0.02, "lens.g2": 0.04,
@@ -146,18 +162,16 @@ def c5():
              "bulge.q": 0.8, "bulge.pa": 10.0,
              "disk.x": 258.0, "disk.y": 255.0, "disk.r": 9.0, "disk.mag": 0.5, "disk.n": 1.0,
              "disk.q": 0.5, "disk.pa": 120.0, "sky.bg": 0.005, "sky.dx": 0.0, "sky.dy": 0.0}
-    # position angles periodic, as Lensed's examples declare them
-    # (examples/full_mock_psf.ini:42,49)
-    priors = {"lens.x": "unif 254 259", "lens.y": "unif 254 259", "lens.r": "unif 35 45",
-              "lens.t": "unif 0.5 1.5", "lens.q": "unif 0.4 1", "lens.pa": "wrap unif 0 180",
+    priors = {"lens.x": "unif 255.5 257.5", "lens.y": "unif 255.5 257.5", "lens.r": "unif 37 43",
+              "lens.t": "unif 0.7 1.1", "lens.q": "unif 0.6 0.9", "lens.pa": "wrap unif 0 180",
               "lens.g1": "unif -0.1 0.1", "lens.g2": "unif -0.1 0.1",
-              "bulge.x": "unif 254 262", "bulge.y": "unif 251 259", "bulge.r": "unif 2 8", "bulge.mag": "unif -5 5",
-              "bulge.n": "unif 1 6", "bulge.q": "unif 0.4 1", "bulge.pa": "wrap unif 0 180",
-              "disk.x": "unif 254 262", "disk.y": "unif 251 259", "disk.r": "unif 4 16", "disk.mag": "unif -5 5",
-              "disk.n": "unif 0.5 2", "disk.q": "unif 0.2 1", "disk.pa": "wrap unif 0 180",
-              "sky.bg": "unif -0.05 0.05"}
+              "bulge.x": "unif 257 259", "bulge.y": "unif 254 256", "bulge.r": "unif 2 6", "bulge.mag": "unif -5 5",
+              "bulge.n": "unif 2.5 4.5", "bulge.q": "unif 0.6 1", "bulge.pa": "wrap unif 0 180",
+              "disk.x": "unif 257 259", "disk.y": "unif 254 256", "disk.r": "unif 6 12", "disk.mag": "unif -5 5",
+              "disk.n": "unif 0.6 1.6", "disk.q": "unif 0.3 0.7", "disk.pa": "wrap unif 0 180",
+              "sky.bg": "unif -0.01 0.02"}
     return Workload("C5", objects, truth, priors, (512, 512), 0.02, psf=moffat_psf(31, 3.0, 3.0), seed=5,
-                    cand_seed=105, ncand=2048)
+                    cand_seed=105, ncand=2048, mag_window=1.5)
 
 
 def c4(nsys=64, ncand=256):
