@@ -41,7 +41,8 @@ VALU_ISSUE_PEAK = 256 * 2.4e9  # wave instructions per second
 REF_FLOP_PER_RAY = 165
 PROFILES = os.path.join(ROOT, "profiles")
 RENDER_PROFILE = "lensed_render_c3_counters.md"
-CONV_PROFILE = "lensed_conv_chi2_c3_counters.md"
+CONV_PROFILE = {"lensed_conv_chi2": "lensed_conv_chi2_c3_counters.md",
+                "lensed_conv_chi2_sweep": "lensed_conv_chi2_sweep_c3_counters.md"}
 
 
 def profile_figure(name, key):
@@ -203,15 +204,18 @@ def render_roofline(kt, tr, K):
                     "SQ_ACTIVE_INST_VALU in the profile)"}
 
 
-def conv_roofline(kt, K, npix=65536, taps=21):
+def conv_roofline(kt, K, conv_rows, npix=65536, taps=21):
     """The convolution + chi^2 kernel against both its bounds: the rank-1
-    separable passes' FP32 work and the model bytes it must read."""
+    separable passes' FP32 work and the model bytes it must read.  The
+    kernel is the one the launch shapes chose: the column sweep
+    (conv_rows 1) or the tile."""
+    kernel = "lensed_conv_chi2_sweep" if conv_rows == 1 else "lensed_conv_chi2"
     s = kt["convolve"] / 1e3
     flop = (2 * 2 * taps + 4) * npix * K  # row pass, column pass, w (m - d)^2
     model_bytes = 4 * npix * K
-    fetch = profile_figure(CONV_PROFILE, "hbm_bytes_per_launch")
+    fetch = profile_figure(CONV_PROFILE[kernel], "hbm_bytes_per_launch")
     hbm = None if fetch is None else fetch * K / 4096
-    return {"kernel": "lensed_conv_chi2", "avg_launch_ms": round(kt["convolve"], 4),
+    return {"kernel": kernel, "avg_launch_ms": round(kt["convolve"], 4),
             "algorithmic_tflops": round(flop / s / 1e12, 3),
             "frac_fp32_nonpacked": round(flop / s / 1e12 / PEAK_FP32_NONPACKED, 4),
             "model_bytes_per_launch": model_bytes,
@@ -549,7 +553,7 @@ def main():
         "stages_note": "set_params includes the emulator and exact-series table kernels; render is "
                        "cull_pass + render_kernel",
         "stages_ms_per_launch_exact": {k: round(v, 4) for k, v in kt_exact.items()},
-        "conv_chi2": conv_roofline(kt, K),
+        "conv_chi2": conv_roofline(kt, K, tuning.get("conv_rows")),
         "accuracy_vs_fp64_oracle": acc64,
         "cpu_baseline": cpu,
         "nested_sampling": nest,

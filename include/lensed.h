@@ -257,7 +257,10 @@ typedef struct {
     double conv_grid;     /* convolution blocks, the same (1) */
     int conv_rows;        /* output rows per thread of the convolution tile (4 rows of threads): 2, 8 or 16
                              (8, or 2 when the launch is too small to fill the device); 16 needs 64 + PSF
-                             height - 1 rows of LDS and falls back to 8 where that passes 64 KiB */
+                             height - 1 rows of LDS and falls back to 8 where that passes 64 KiB; or 1, the
+                             column sweep: one wave per 64-column strip of an image runs down the strip,
+                             for launches whose systems all have a rank-1 separable PSF at most 65 wide and
+                             at most 512 rows (otherwise the default tile) */
     int conv_order;       /* convolution items: 1 candidate-fastest over the grid, 2 each block a run of one
                              candidate's tiles down 64-column strips within its XCD's share (1) */
     /* read back by lensed_get_tuning() after an autotuned launch, else 0: */

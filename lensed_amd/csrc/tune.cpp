@@ -9,12 +9,15 @@
 // Search, coordinate by coordinate from the defaults: the render stage
 // (which includes the pixel cull) over the list/walk switch and the render
 // and cull grid sizes; then the convolution stage over the tile height and
-// the item order together, then its grid size.  A stage keeps a setting
+// the item order together, the column sweep, then the grid size (the
+// sweep's grid is its item count, so the grid knob leaves it unchanged).  A stage keeps a setting
 // only when it beats the stage's best time so far by more than the timing
 // noise; at the end the chosen shapes must beat the defaults over the whole
 // chain, timed alternately, or the defaults stay.  Choices are cached on
 // disk per device, program and layout, next to the program cache.
 #include "tune.h"
+
+#include "../kernels/desc.h"
 
 #include <sys/stat.h>
 #include <unistd.h>
@@ -120,7 +123,9 @@ int autotune(const TuneTimer& timer, bool culls, lensed_tuning* best) {
             c.cull_grid = g;
             if (try_render(c)) return -1;
         }
-    // convolution stage: tile height with item order, then the grid
+    // convolution stage: tile height with item order, the column sweep
+    // (layouts it does not serve build the default tile, which then times
+    // as the default), then the grid
     for (int rows : {8, 16, 2})
         for (int order : {1, 2}) {
             lensed_tuning c = cur;
@@ -128,6 +133,11 @@ int autotune(const TuneTimer& timer, bool culls, lensed_tuning* best) {
             c.conv_order = order;
             if (try_conv(c)) return -1;
         }
+    {
+        lensed_tuning c = cur;
+        c.conv_rows = LENSED_CONV_SWEEP;
+        if (try_conv(c)) return -1;
+    }
     for (double g : {0.5, 2.0, 4.0}) {
         lensed_tuning c = cur;
         c.conv_grid = g;

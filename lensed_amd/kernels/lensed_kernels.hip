@@ -646,13 +646,16 @@ __device__ inline void lensed_conv_sep_rank(const float* __restrict__ cf, float*
             float4 f = src[q];
             v[4 * q] = f.x; v[4 * q + 1] = f.y; v[4 * q + 2] = f.z; v[4 * q + 3] = f.w;
         }
+        // the chain's order written out (first tap a product, then fused
+        // multiply-adds in tap order): contraction may not pick another
+        // pairing, which the column sweep repeats bit for bit
         float o[NX];
 #pragma unroll
-        for (int x = 0; x < NX; ++x) o[x] = 0.0f;
+        for (int x = 0; x < NX; ++x) o[x] = rf[0] * v[x + PW - 1];
 #pragma unroll
-        for (int i = 0; i < PW; ++i)
+        for (int i = 1; i < PW; ++i)
 #pragma unroll
-            for (int x = 0; x < NX; ++x) o[x] += rf[i] * v[x + PW - 1 - i];
+            for (int x = 0; x < NX; ++x) o[x] = __builtin_fmaf(rf[i], v[x + PW - 1 - i], o[x]);
         *(float4*)(tmp + r * TS + x0) = make_float4(o[0], o[1], o[2], o[3]);
     }
     __syncthreads();
@@ -662,7 +665,7 @@ __device__ inline void lensed_conv_sep_rank(const float* __restrict__ cf, float*
 #pragma unroll
     for (int j = 0; j < PH; ++j)
 #pragma unroll
-        for (int r = 0; r < RY; ++r) acc[r] += cf[j] * col[r + PH - 1 - j];
+        for (int r = 0; r < RY; ++r) acc[r] = __builtin_fmaf(cf[j], col[r + PH - 1 - j], acc[r]);
     __syncthreads();
 }
 
@@ -896,6 +899,231 @@ extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) __attribute__((amdgpu_wa
 {
     lensed_conv_chi2_body<LENSED_CONV_RY_TALL>(sys, nsys, nitems, model, image, weight, psf, partials,
                                                conv_out, chi2_out, trace, order);
+}
+
+// ------------------------------------------------------ conv+chi2, sweep
+// Rank-1 separable PSFs, images up to 512 rows: one wave per 64-column strip
+// of one candidate's image sweeps the strip from top to bottom.  Per input
+// row, the row pass of the strip's 64 columns, from the row and its PSF halo
+// staged in the wave's LDS row, goes into a window of the last PH row-pass
+// values per lane (registers; the sweep is unrolled PH rows at a time, so
+// every window index is static).  Per output row, the column pass over the
+// window, the chi^2 term, and every 8 rows the compensated block sum.  Each
+// output is the tile kernel's FMA chain (row taps in order, then column taps
+// in order, clamp-to-edge rows and columns), and the chi^2 blocks are the
+// same 8 x 64 blocks with the same lane order, so both kernels give the same
+// bits.  Rows above and below the image are its first and last rows, whose
+// row pass is computed once.  After a pixel cull, an input row whose 8-row
+// band is dark over the strip and its halo is exactly 0: it is neither
+// loaded nor row-passed; an output row whose whole window is dark is exactly
+// 0 without the column pass.  Nothing is reread: each row of the model is
+// loaded once per strip (its halo columns also by the neighbouring strips,
+// which run beside it in the same XCD), against the tile kernel's halo rows.
+#define LENSED_SWEEP_ROW 128  // floats per wave's staged row: 64 + PW - 1 for PW <= 65
+
+template <int PW, int PH, bool OUT>
+__device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, int strip,
+                                              const float* __restrict__ model, const float* __restrict__ image,
+                                              const float* __restrict__ weight, const float* __restrict__ psf,
+                                              double* __restrict__ partials, float* __restrict__ conv_out,
+                                              float* __restrict__ chi2_out, const int* __restrict__ trace,
+                                              float* lrow)
+{
+#pragma clang fp reassociate(off)
+    constexpr int A = PH - 1 - PH / 2;  // output row y takes row-pass rows y + A down to y + A - (PH - 1)
+    constexpr int LW = 64 + PW - 1;     // the staged row: the strip and its PSF halo
+    static_assert(LW <= LENSED_SWEEP_ROW, "PSF wider than the staged row");
+    const int lane = (int)threadIdx.x & 63;
+    const int W = d.width, H = d.height;
+    const int x0 = strip * 64, gx = x0 + lane;
+    const float* m = model + d.model_off + (long long)cand * d.npix;
+    const float* img = image + d.image_off;
+    const float* wgt = weight + d.image_off;
+    const float* cf = psf + d.sep_off;  // column factor [PH], then row factor [PW]
+    const float* rf = cf + PH;
+    // the two staged columns of this lane (lane and lane + 64 of the row),
+    // clamped to the image as the tile load clamps; the second is loaded by
+    // every lane (a clamped column, cached) and stored by those it belongs to
+    const unsigned ca = (unsigned)min(max(x0 - PW / 2 + lane, 0), W - 1);
+    const bool has_b = lane + 64 < LW;
+    const unsigned cb = (unsigned)min(max(x0 - PW / 2 + lane + 64, 0), W - 1);
+    const unsigned xd = (unsigned)min(gx, W - 1);  // data column (lanes past the image edge keep nothing)
+    // lit 8-row bands (bit b: rows 8b .. 8b + 7) over the columns the strip's
+    // row pass reads; lane b ORs the flags of band b's squares
+    unsigned long long lit = ~0ull;
+    if (trace && d.cull) {
+        const int c0 = max(x0 - PW / 2, 0), c1 = min(x0 + 63 + (PW - 1 - PW / 2), W - 1);
+        int any = 0;
+        if (lane * 8 < H) {
+            const int ntx = (W + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
+            // squares of a render item: x offset (w & 1) * 8, y offset (w >> 1) * 8
+            const int* f = trace + 4 * ((long long)d.item_r + (long long)cand * d.ntile_r + (lane >> 1) * ntx);
+            const int sub = (lane & 1) * 2;
+            for (int sx = c0 >> 3; sx <= (c1 >> 3); ++sx) any |= f[4 * (sx >> 1) + (sx & 1) + sub];
+        }
+        lit = __ballot(any != 0);
+    }
+    double* part = partials + d.partial_off + (long long)cand * d.npart + strip;
+    float rfs[PW], cfs[PH];
+#pragma unroll
+    for (int i = 0; i < PW; ++i) rfs[i] = rf[i];
+#pragma unroll
+    for (int j = 0; j < PH; ++j) cfs[j] = cf[j];
+    auto row_lit = [&](int y) { return y < H && ((lit >> (y >> 3)) & 1ull); };
+    // the model row y for the staged columns; dark rows and rows past the
+    // image read row 0 instead (cached; the values are not used), so every
+    // iteration issues the same loads and the waits stay exact
+    auto fetch = [&](int y, float& va, float& vb) {
+        const float* src = m + (long long)(row_lit(y) ? y : 0) * W;
+        va = src[ca];
+        vb = src[cb];
+    };
+    // data and weight of output row y (rows outside the image read its edge, unused)
+    auto fetch_data = [&](int y, float& dv, float& wv) {
+        const long long o = (long long)min(max(y, 0), H - 1) * W + xd;
+        dv = img[o];
+        wv = wgt[o];
+    };
+    float win[PH];
+    float last = 0.0f;  // the row pass of the last image row (the rows below it)
+    float s = 0.0f, c = 0.0f;
+    // the loads of the next four iterations (model row yi, data row yi - A),
+    // issued in iteration order: the oldest are consumed first, so four
+    // iterations' loads stay in flight under the counter waits
+    float a0, b0, d0, w0, a1, b1, d1, w1, a2, b2, d2, w2, a3, b3, d3, w3;
+    fetch(0, a0, b0);
+    fetch_data(-A, d0, w0);
+    fetch(1, a1, b1);
+    fetch_data(1 - A, d1, w1);
+    fetch(2, a2, b2);
+    fetch_data(2 - A, d2, w2);
+    fetch(3, a3, b3);
+    fetch_data(3 - A, d3, w3);
+    for (int y0 = 0; y0 < H + A; y0 += PH) {
+#pragma unroll
+        for (int k = 0; k < PH; ++k) {
+            const int yi = y0 + k;  // input row (wave-uniform)
+            if (yi < H + A) {
+                const float va = a0, vb = b0, dat = d0, wt = w0;
+                a0 = a1, b0 = b1, d0 = d1, w0 = w1;
+                a1 = a2, b1 = b2, d1 = d2, w1 = w2;
+                a2 = a3, b2 = b3, d2 = d3, w2 = w3;
+                fetch(yi + 4, a3, b3);
+                fetch_data(yi + 4 - A, d3, w3);
+                const int y = yi - A;  // output row
+                float r = last;
+                if (yi < H) {
+                    r = 0.0f;
+                    if (row_lit(yi)) {
+                        // the tile kernel's chain: the first tap a product, then
+                        // fused multiply-adds in tap order (written out, so that
+                        // contraction cannot pair the terms differently)
+                        // one wave writes and reads the row, and its LDS accesses
+                        // complete in order; the compiler must keep that order
+                        // (it sees one lane's distinct addresses, not the others')
+                        asm volatile("" ::: "memory");
+                        lrow[lane] = va;
+                        if (has_b) lrow[lane + 64] = vb;
+                        asm volatile("" ::: "memory");
+                        r = rfs[0] * lrow[lane + PW - 1];
+#pragma unroll
+                        for (int i = 1; i < PW; ++i) r = __builtin_fmaf(rfs[i], lrow[lane + PW - 1 - i], r);
+                    }
+                    last = r;
+                    if (yi == 0) {
+#pragma unroll
+                        for (int q = 0; q < PH; ++q) win[q] = r;  // rows above the image are row 0
+                    }
+                }
+                win[k] = r;
+                if (y >= 0) {
+                    // window rows yi - PH + 1 .. yi, clamped to the image
+                    const int lo = max(yi - PH + 1, 0) >> 3, hi = min(yi, H - 1) >> 3;
+                    const unsigned long long span = ((2ull << (hi - lo)) - 1) << lo;
+                    float acc = 0.0f;
+                    if (lit & span) {
+#pragma unroll
+                        for (int j = 0; j < PH; ++j) acc = __builtin_fmaf(cfs[j], win[(k - j + PH) % PH], acc);
+                    }
+                    const float dm = acc - dat;
+                    const float chi = gx < W ? wt * dm * dm : 0.0f;
+                    if (OUT && gx < W) {
+                        const long long kk = d.model_off + (long long)cand * d.npix + (long long)y * W + gx;
+                        if (conv_out) conv_out[kk] = acc;
+                        if (chi2_out) chi2_out[kk] = chi;
+                    }
+                    lensed_two_sum_add(s, c, chi);
+                    if ((y & 7) == 7 || y == H - 1) {
+                        // rows past the last one count as 0, as in the tile kernel's last block
+                        for (int z = (y & 7) + 1; z < 8; ++z) lensed_two_sum_add(s, c, 0.0f);
+                        const double blk = lensed_wave_csum(s, c);
+                        if (lane == 0) part[(long long)(y >> 3) * d.tiles_x] = blk;
+                        s = c = 0.0f;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// One wave per item (a 64-column strip of one candidate), four per block.
+// Blocks go to the eight XCDs in turn; XCD x takes the x-th contiguous share
+// of the blocks, so the strips of a candidate run side by side in one L2,
+// which holds the halo columns that neighbouring strips both read.  The
+// dumper's planes have a kernel of their own (the stores would otherwise
+// share the loglike kernel's registers and make it spill).
+template <bool OUT>
+__device__ inline void lensed_conv_sweep_body(const lensed_desc* __restrict__ sys, int nsys, int nitems,
+                                              const float* __restrict__ model, const float* __restrict__ image,
+                                              const float* __restrict__ weight, const float* __restrict__ psf,
+                                              double* __restrict__ partials, float* __restrict__ conv_out,
+                                              float* __restrict__ chi2_out, const int* __restrict__ trace)
+{
+    __shared__ float rows[4 * LENSED_SWEEP_ROW];
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    int blk = b;
+    if (nb >= 8) {
+        const int x = b & 7, q = nb >> 3, r = nb & 7;
+        blk = x * q + min(x, r) + (b >> 3);
+    }
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);  // wave-uniform, and known so
+    const int item = blk * 4 + wave;
+    if (item >= nitems) return;
+    const int s = lensed_find_sys<1>(sys, nsys, item);
+    const lensed_desc& d = sys[s];
+    const int local = item - d.item_c;
+    const int cand = local / d.tiles_x, strip = local - cand * d.tiles_x;
+    float* lrow = rows + wave * LENSED_SWEEP_ROW;
+#define LENSED_SWEEP_CASE(PW_, PH_, SEP_)                                                                 \
+    if (SEP_ && d.psfw == PW_ && d.psfh == PH_) {                                                       \
+        lensed_conv_sweep_item<PW_, PH_, OUT>(d, cand, strip, model, image, weight, psf, partials,      \
+                                              conv_out, chi2_out, trace, lrow);                         \
+        return;                                                                                         \
+    }
+    LENSED_FOR_PSFS(LENSED_SWEEP_CASE)
+#undef LENSED_SWEEP_CASE
+}
+
+extern "C" __global__ __launch_bounds__(256) void lensed_conv_chi2_sweep(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems,
+    const float* __restrict__ model, const float* __restrict__ image,
+    const float* __restrict__ weight, const float* __restrict__ psf,
+    double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out,
+    const int* __restrict__ trace, int order)
+{
+    lensed_conv_sweep_body<false>(sys, nsys, nitems, model, image, weight, psf, partials, nullptr, nullptr, trace);
+    (void)conv_out, (void)chi2_out, (void)order;
+}
+
+extern "C" __global__ __launch_bounds__(256) void lensed_conv_chi2_sweep_out(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems,
+    const float* __restrict__ model, const float* __restrict__ image,
+    const float* __restrict__ weight, const float* __restrict__ psf,
+    double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out,
+    const int* __restrict__ trace, int order)
+{
+    lensed_conv_sweep_body<true>(sys, nsys, nitems, model, image, weight, psf, partials, conv_out, chi2_out, trace);
+    (void)order;
 }
 
 // ---------------------------------------------------------------- finalize

@@ -230,16 +230,18 @@ def test_walk_over_culled_items_is_exact(c3):
 
 def test_launch_shapes_keep_results_bit_identical(c3):
     """Every launch-shape knob (the autotuner's search space) leaves the logL
-    of every candidate unchanged, bit for bit: tile heights 2, 8 and 16
-    rows per thread, both convolution item orders, grids from half to twice
-    the resident count, list and walk."""
+    of every candidate unchanged, bit for bit: the column sweep (the default
+    of this culled layout), tile heights 2, 8 and 16 rows per thread, both
+    convolution item orders, grids from half to twice the resident count,
+    list and walk."""
     wl, pids = c3
     cubes = np.random.default_rng(41).random((1024, 14))
     with Context(wl.spec(), max_batch=1024, flags=LENSED_TABLE | LENSED_GRAPH) as ctx:
         ref, _ = ctx.loglike_batch(cubes)
         base = ctx.tuning()
-        assert base["conv_rows"] == 8 and base["conv_order"] == 1 and base["list_fraction"] == 0.75
-        for knobs in (dict(conv_rows=2), dict(conv_rows=16), dict(conv_order=2), dict(conv_rows=16, conv_order=2),
+        assert base["conv_rows"] == 1 and base["conv_order"] == 1 and base["list_fraction"] == 0.75
+        for knobs in (dict(conv_rows=8), dict(conv_rows=2), dict(conv_rows=16), dict(conv_rows=8, conv_order=2),
+                      dict(conv_rows=16, conv_order=2),
                       dict(render_grid=0.5, cull_grid=2.0, conv_grid=2.0), dict(render_grid=2.0, conv_grid=0.5),
                       dict(list_fraction=2.0), dict(list_fraction=-1.0, conv_order=2)):
             ctx.set_tuning(**knobs)
