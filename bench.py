@@ -389,14 +389,10 @@ def nested_compare(wl, pids, K, nlive=400):
             "logz_difference_in_sigma": round(abs(rb.logz - lz.value) / math.hypot(rb.logz_err, lze.value), 3)}
 
 
-def c4_launches(launches=3, flags_extra=0):
-    """The uneven-batch workload (64 systems of 64..512 px, SIE or EPL +
-    Sersic, 15 x 15 or 31 x 31 PSFs, 256 prior draws per system, one launch):
-    ms per launch and per stage, on the table path with the render walking
-    the pixel cull's list (list_fraction 2) and walking the items (-1), and
-    without culling."""
+def c4_workload():
+    """The uneven-batch workload's 64 systems with mock data, and 256 seeded
+    prior draws per system (physical parameters)."""
     from lensed_amd import workloads as W
-    from lensed_amd._lib import LENSED_NO_CULL, LENSED_PROFILE, LENSED_TABLE
     from lensed_amd.lensed import Context, LensedObject, SystemSpec
     wls = W.c4()
     for wl in wls:
@@ -412,6 +408,28 @@ def c4_launches(launches=3, flags_extra=0):
         wl.pids = pids
     rng = np.random.default_rng(404)
     phys = [np.array([[rng.uniform(*wl.prior_range(k)) for k in wl.pids] for _ in range(256)]) for wl in wls]
+    return wls, phys
+
+
+def stage_ms(ctx, run, launches):
+    """ms per launch of each profiled stage over `launches` runs after one."""
+    run()
+    c0, m0 = ctx.profile()
+    for _ in range(launches):
+        run()
+    c1, m1 = ctx.profile()
+    return {k: round((m1[k] - m0[k]) / (c1 - c0), 4) for k in m1}
+
+
+def c4_launches(launches=3, flags_extra=0):
+    """The uneven-batch workload (64 systems of 64..512 px, SIE or EPL +
+    Sersic, 15 x 15 or 31 x 31 PSFs, 256 prior draws per system, one launch):
+    ms per launch and per stage, on the table path with the render walking
+    the pixel cull's list (list_fraction 2) and walking the items (-1), and
+    without culling."""
+    from lensed_amd._lib import LENSED_NO_CULL, LENSED_PROFILE, LENSED_TABLE
+    from lensed_amd.lensed import Context
+    wls, phys = c4_workload()
     out = {}
     for name, extra, frac in (("list", 0, 2.0), ("walk", 0, -1.0), ("no_cull", LENSED_NO_CULL, 0.0)):
         with Context([wl.spec() for wl in wls], max_batch=256,
@@ -433,6 +451,32 @@ def c4_launches(launches=3, flags_extra=0):
     return {"workload": f"C4: 64 systems, {npix / 1e6:.2f} Mpx, 256 prior draws each, EPL table path",
             "launches": launches, "paths": out,
             "evals_per_s": {k: round(64 * 256 / (v["ms_per_launch"] / 1e3), 1) for k, v in out.items()}}
+
+
+def conv_shapes(launches=5):
+    """The convolution stage (ms per launch, hipEvents) under the 32-row tile
+    (conv_rows 8) and the column sweep (conv_rows 1): C3 at 4096 prior draws
+    per launch (culled) and C4 (64 systems, not culled; the sweep's rows all
+    lit).  The tile's launches come first, then the sweep's."""
+    from lensed_amd._lib import LENSED_PROFILE, LENSED_TABLE
+    from lensed_amd.lensed import Context
+    K = 4096
+    wl, _ = make_workload(K)
+    out = {"C3": {}, "C4": {}}
+    with Context(wl.spec(), max_batch=K, flags=LENSED_TABLE | LENSED_PROFILE) as ctx:
+        sets = draws(wl, ctx.ndims(), K, launches + 1)
+        for rows in (8, 1):
+            ctx.set_tuning(conv_rows=rows)
+            it = iter(sets)
+            out["C3"][f"conv_rows_{rows}"] = stage_ms(ctx, lambda: ctx.loglike_batch(next(it)), launches)["convolve"]
+    wls, phys = c4_workload()
+    with Context([w.spec() for w in wls], max_batch=256, flags=LENSED_TABLE | LENSED_PROFILE) as ctx:
+        for rows in (8, 1):
+            ctx.set_tuning(conv_rows=rows)
+            out["C4"][f"conv_rows_{rows}"] = stage_ms(ctx, lambda: ctx.loglike_phys(phys), 2)["convolve"]
+    return {"convolve_ms_per_launch": out,
+            "note": "conv_rows 8: the 32-row tile; conv_rows 1: the column sweep (the default where every "
+                    "system is culled)"}
 
 
 def copy_probe(mib, reps=3):
@@ -461,6 +505,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="time the default launch shapes")
     ap.add_argument("--c4", action="store_true",
                     help="only time the uneven 64-system workload (render list, item walk, no cull) and exit")
+    ap.add_argument("--conv-shapes", action="store_true",
+                    help="only time the convolution under the tile and the column sweep (C3, C4) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (byte-counter calibration) and exit")
     args = ap.parse_args()
@@ -475,6 +521,9 @@ def main():
         return
     if args.c4:
         print(json.dumps(c4_launches()))
+        return
+    if args.conv_shapes:
+        print(json.dumps(conv_shapes()))
         return
     from lensed_amd._lib import LENSED_AUTOTUNE, LENSED_EMULATE, LENSED_GRAPH, LENSED_TABLE
     global TUNED

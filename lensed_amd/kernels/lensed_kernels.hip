@@ -208,12 +208,23 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
     int first, end;
     lensed_xcd_run(listed ? live[0] / 4 : nitems, first, end);
     int staged = -1;  // first item of the candidate whose table is in LDS
+    // a list entry is read one entry ahead: its loads are in flight while the
+    // block renders the current one, instead of stalling every entry
+    int e_first = 0, e_mine = 0;
+    if (listed && first < end) {
+        e_first = live[1 + 4 * first];
+        e_mine = live[1 + 4 * first + wave];
+    }
     for (int j = first; j < end; ++j) {
         int item = j, mine = j, sq = wave;
         bool on = true;
         if (listed) {
-            item = live[1 + 4 * j] >> 2;  // the entry's candidate is that of its first square
-            const int v = __builtin_amdgcn_readfirstlane(live[1 + 4 * j + wave]);
+            item = __builtin_amdgcn_readfirstlane(e_first) >> 2;  // the entry's candidate: its first square's
+            const int v = __builtin_amdgcn_readfirstlane(e_mine);
+            if (j + 1 < end) {
+                e_first = live[1 + 4 * (j + 1)];
+                e_mine = live[1 + 4 * (j + 1) + wave];
+            }
             on = v >= 0;
             mine = on ? v >> 2 : item;
             sq = on ? v & 3 : 0;
@@ -516,11 +527,32 @@ __device__ inline void lensed_two_sum_add(float& s, float& c, float x) {
     c += e;
 }
 
+// the value of lane ^ o: a swizzle within each 32-lane half for o < 32
+// (bitmask mode: and 0x1f, or 0, xor o), a permute across the halves for 32;
+// no lane arithmetic, unlike a general shuffle
+template <int O>
+__device__ inline float lensed_lane_xor(float v) {
+    if (O == 32) {
+        const int lane = (int)__lane_id();
+        return __int_as_float(__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, __float_as_int(v)));
+    }
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (O << 10) | 0x1f));
+}
+
 __device__ inline double lensed_wave_csum(float s, float c) {
 #pragma clang fp reassociate(off) contract(off)
-    for (int o = 32; o > 0; o >>= 1) {
-        float s2 = __shfl_xor(s, o, 64);
-        float c2 = __shfl_xor(c, o, 64);
+#pragma unroll
+    for (int lv = 0; lv < 6; ++lv) {
+        const int o = 32 >> lv;
+        float s2, c2;
+        switch (o) {
+            case 32: s2 = lensed_lane_xor<32>(s), c2 = lensed_lane_xor<32>(c); break;
+            case 16: s2 = lensed_lane_xor<16>(s), c2 = lensed_lane_xor<16>(c); break;
+            case 8: s2 = lensed_lane_xor<8>(s), c2 = lensed_lane_xor<8>(c); break;
+            case 4: s2 = lensed_lane_xor<4>(s), c2 = lensed_lane_xor<4>(c); break;
+            case 2: s2 = lensed_lane_xor<2>(s), c2 = lensed_lane_xor<2>(c); break;
+            default: s2 = lensed_lane_xor<1>(s), c2 = lensed_lane_xor<1>(c); break;
+        }
         float t = s + s2;  // TwoSum(s, s2): t + e == s + s2 exactly
         float bb = t - s;
         float e = (s - (t - bb)) + (s2 - bb);
@@ -944,10 +976,10 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
     // the two staged columns of this lane (lane and lane + 64 of the row),
     // clamped to the image as the tile load clamps; the second is loaded by
     // every lane (a clamped column, cached) and stored by those it belongs to
-    const unsigned ca = (unsigned)min(max(x0 - PW / 2 + lane, 0), W - 1);
+    const unsigned ca = 4u * (unsigned)min(max(x0 - PW / 2 + lane, 0), W - 1);  // byte offsets in a row
     const bool has_b = lane + 64 < LW;
-    const unsigned cb = (unsigned)min(max(x0 - PW / 2 + lane + 64, 0), W - 1);
-    const unsigned xd = (unsigned)min(gx, W - 1);  // data column (lanes past the image edge keep nothing)
+    const unsigned cb = 4u * (unsigned)min(max(x0 - PW / 2 + lane + 64, 0), W - 1);
+    const unsigned xd = 4u * (unsigned)min(gx, W - 1);  // data column (lanes past the image edge keep nothing)
     // lit 8-row bands (bit b: rows 8b .. 8b + 7) over the columns the strip's
     // row pass reads; lane b ORs the flags of band b's squares
     unsigned long long lit = ~0ull;
@@ -970,35 +1002,40 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
 #pragma unroll
     for (int j = 0; j < PH; ++j) cfs[j] = cf[j];
     auto row_lit = [&](int y) { return y < H && ((lit >> (y >> 3)) & 1ull); };
-    // the model row y for the staged columns; dark rows and rows past the
-    // image read row 0 instead (cached; the values are not used), so every
-    // iteration issues the same loads and the waits stay exact
-    auto fetch = [&](int y, float& va, float& vb) {
-        const float* src = m + (long long)(row_lit(y) ? y : 0) * W;
-        va = src[ca];
-        vb = src[cb];
-    };
-    // data and weight of output row y (rows outside the image read its edge, unused)
-    auto fetch_data = [&](int y, float& dv, float& wv) {
-        const long long o = (long long)min(max(y, 0), H - 1) * W + xd;
-        dv = img[o];
-        wv = wgt[o];
+    const bool last_lit = row_lit(H - 1);  // the rows below the image repeat the last one
+    // Loads, issued four iterations ahead in iteration order so that the
+    // counter waits keep four iterations in flight; every address is a fixed
+    // base plus a wave-uniform row offset (kept by increments) plus the
+    // lane's byte offset.  Model row yf of the staged columns, or row 0 where
+    // yf is dark or past the image (cached; the values are not used), so every
+    // iteration issues the same loads; data and weight of output row yd,
+    // clamped to the image (rows outside it are not used).
+    const char* mb = (const char*)m;
+    const char* db = (const char*)img;
+    const char* wb = (const char*)wgt;
+    unsigned moff = 0, doff = 0;  // byte offsets of the next fetches' rows
+    int yf = 0, yd = -A;          // and the rows themselves
+    auto at = [](const char* base, unsigned off) { return *(const float*)(base + off); };
+    auto fetch = [&](float& va, float& vb, float& dv, float& wv) {
+        const unsigned mo = row_lit(yf) ? moff : 0u;
+        va = at(mb, mo + ca);
+        vb = at(mb, mo + cb);
+        dv = at(db, doff + xd);
+        wv = at(wb, doff + xd);
+        moff += 4u * (unsigned)W;
+        ++yf;
+        if (yd >= 0 && yd < H - 1) doff += 4u * (unsigned)W;
+        ++yd;
     };
     float win[PH];
     float last = 0.0f;  // the row pass of the last image row (the rows below it)
     float s = 0.0f, c = 0.0f;
-    // the loads of the next four iterations (model row yi, data row yi - A),
-    // issued in iteration order: the oldest are consumed first, so four
-    // iterations' loads stay in flight under the counter waits
+    int age = PH;  // input rows since the last lit one (a window is lit while under PH)
     float a0, b0, d0, w0, a1, b1, d1, w1, a2, b2, d2, w2, a3, b3, d3, w3;
-    fetch(0, a0, b0);
-    fetch_data(-A, d0, w0);
-    fetch(1, a1, b1);
-    fetch_data(1 - A, d1, w1);
-    fetch(2, a2, b2);
-    fetch_data(2 - A, d2, w2);
-    fetch(3, a3, b3);
-    fetch_data(3 - A, d3, w3);
+    fetch(a0, b0, d0, w0);
+    fetch(a1, b1, d1, w1);
+    fetch(a2, b2, d2, w2);
+    fetch(a3, b3, d3, w3);
     for (int y0 = 0; y0 < H + A; y0 += PH) {
 #pragma unroll
         for (int k = 0; k < PH; ++k) {
@@ -1008,13 +1045,14 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
                 a0 = a1, b0 = b1, d0 = d1, w0 = w1;
                 a1 = a2, b1 = b2, d1 = d2, w1 = w2;
                 a2 = a3, b2 = b3, d2 = d3, w2 = w3;
-                fetch(yi + 4, a3, b3);
-                fetch_data(yi + 4 - A, d3, w3);
+                fetch(a3, b3, d3, w3);
+                const bool lit_in = yi < H ? row_lit(yi) : last_lit;
                 const int y = yi - A;  // output row
+                age = lit_in ? 0 : age + 1;
                 float r = last;
                 if (yi < H) {
                     r = 0.0f;
-                    if (row_lit(yi)) {
+                    if (lit_in) {
                         // the tile kernel's chain: the first tap a product, then
                         // fused multiply-adds in tap order (written out, so that
                         // contraction cannot pair the terms differently)
@@ -1037,11 +1075,10 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
                 }
                 win[k] = r;
                 if (y >= 0) {
-                    // window rows yi - PH + 1 .. yi, clamped to the image
-                    const int lo = max(yi - PH + 1, 0) >> 3, hi = min(yi, H - 1) >> 3;
-                    const unsigned long long span = ((2ull << (hi - lo)) - 1) << lo;
+                    // the window (input rows yi - PH + 1 .. yi, clamped) holds a
+                    // lit row unless the last lit one is PH or more rows back
                     float acc = 0.0f;
-                    if (lit & span) {
+                    if (age < PH) {
 #pragma unroll
                         for (int j = 0; j < PH; ++j) acc = __builtin_fmaf(cfs[j], win[(k - j + PH) % PH], acc);
                     }
