@@ -128,9 +128,9 @@ inline float2 lensed_frame_trace(float4 pre, float2 n0, float2 n1, float2 v) {
 // skipped).  Norms are Frobenius norms, which bound the operator norms.  In
 // the source's frame u_s = T (y - c_s) every ray is then within |T| times
 // that of the centre ray's u_s, and the pixel is skipped when even the
-// nearest is beyond the source's cutoff (frame_cutoff) by a margin of 1:
-// the FP32 rounding of the traced positions and the table's interpolation
-// error stay far below that.
+// nearest is beyond the source's cutoff (frame_cutoff) by a margin of 1/16
+// of a frame unit: the FP32 rounding of the traced positions and of the
+// bound, and the table's interpolation error, come to ~1e-5 of a unit.
 template <class D>
 inline float lensed_epl_slope_bound(const D* self) {
     float a = 1.0f, b = 1.0f;
@@ -145,13 +145,13 @@ inline float lensed_epl_slope_bound(const D* self) {
 inline float lensed_frob(mat22 m) { return sqrtf(m.x * m.x + m.y * m.y + m.z * m.z + m.w * m.w); }
 
 // the candidate's constants, once (set_params): cc[0] = (n0, n1), cc[1] =
-// (|F|, |bwd| (|1-t| A + B) |T|, |I - S| |T|, cutoff + 1), cc[2].x = t
+// (|F|, |bwd| (|1-t| A + B) |T|, |I - S| |T|, cutoff + 1/16), cc[2].x = t
 template <class D>
 inline void lensed_frame_cull_consts(const D* self, float2 n0, float2 n1, float tnorm, float lnorm, float cutoff,
                                      float4* cc) {
     cc[0] = float4(n0.x, n0.y, n1.x, n1.y);
     cc[1] = float4(lensed_frob(self->fwd), lensed_frob(self->bwd) * lensed_epl_slope_bound(self) * tnorm,
-                   lnorm * tnorm, cutoff + 1.0f);
+                   lnorm * tnorm, cutoff + 0.0625f);
     cc[2] = float4(1.0f - 2.0f * self->hexp, 0.0f, 0.0f, 0.0f);
 }
 

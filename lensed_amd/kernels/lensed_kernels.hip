@@ -208,23 +208,12 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
     int first, end;
     lensed_xcd_run(listed ? live[0] / 4 : nitems, first, end);
     int staged = -1;  // first item of the candidate whose table is in LDS
-    // a list entry is read one entry ahead: its loads are in flight while the
-    // block renders the current one, instead of stalling every entry
-    int e_first = 0, e_mine = 0;
-    if (listed && first < end) {
-        e_first = live[1 + 4 * first];
-        e_mine = live[1 + 4 * first + wave];
-    }
     for (int j = first; j < end; ++j) {
         int item = j, mine = j, sq = wave;
         bool on = true;
         if (listed) {
-            item = __builtin_amdgcn_readfirstlane(e_first) >> 2;  // the entry's candidate: its first square's
-            const int v = __builtin_amdgcn_readfirstlane(e_mine);
-            if (j + 1 < end) {
-                e_first = live[1 + 4 * (j + 1)];
-                e_mine = live[1 + 4 * (j + 1) + wave];
-            }
+            item = live[1 + 4 * j] >> 2;  // the entry's candidate is that of its first square
+            const int v = __builtin_amdgcn_readfirstlane(live[1 + 4 * j + wave]);
             on = v >= 0;
             mine = on ? v >> 2 : item;
             sq = on ? v & 3 : 0;

@@ -41,10 +41,12 @@ static float frame_profile(local data* this, float2 u)
     return this->peak*exp(-DEVAUC_BN*sqrt(sqrt(s)));
 }
 
-// |u| beyond which frame_profile is exactly 0 in FP32 (below 2^-160)
+// |u| beyond which frame_profile is below 2^-127 (127 ln 2 = 88.03): a
+// subnormal, which the kernels' FP32 arithmetic (subnormals flushed, as the
+// reference's -cl-denorms-are-zero) takes as 0
 static float frame_cutoff(local data* this)
 {
-    float a = fmax(log(this->peak) + 111, 0.0f)/DEVAUC_BN;
+    float a = fmax(log(this->peak) + 88.03f, 0.0f)/DEVAUC_BN;
     return this->rs*(a*a)*(a*a);
 }
 

@@ -35,10 +35,12 @@ static float frame_profile(local data* this, float2 u)
     return this->peak*exp(-length(u)/this->rs);
 }
 
-// |u| beyond which frame_profile is exactly 0 in FP32 (below 2^-160)
+// |u| beyond which frame_profile is below 2^-127 (127 ln 2 = 88.03): a
+// subnormal, which the kernels' FP32 arithmetic (subnormals flushed, as the
+// reference's -cl-denorms-are-zero) takes as 0
 static float frame_cutoff(local data* this)
 {
-    return this->rs*fmax(log(this->peak) + 111, 0.0f);
+    return this->rs*fmax(log(this->peak) + 88.03f, 0.0f);
 }
 
 static float brightness(local data* this, float2 x)

@@ -35,10 +35,12 @@ static float frame_profile(local data* this, float2 u)
     return exp2(fma(-u.x, u.x, fma(-u.y, u.y, this->lpk)));
 }
 
-// |u| beyond which frame_profile is exactly 0 in FP32 (its exponent below -160)
+// |u| beyond which frame_profile is below 2^-127: a subnormal, which the
+// kernels' FP32 arithmetic (subnormals flushed, as the reference's
+// -cl-denorms-are-zero) takes as 0, so the ray adds exactly nothing
 static float frame_cutoff(local data* this)
 {
-    return sqrt(fmax(this->lpk + 160, 0.0f));
+    return sqrt(fmax(this->lpk + 127, 0.0f));
 }
 
 static float brightness(local data* this, float2 x)

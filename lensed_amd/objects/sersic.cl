@@ -41,10 +41,12 @@ static float frame_profile(local data* this, float2 u)
     return exp2(this->lnorm - 1.44269504088896340736f*exp2(this->lscale + this->hinv*log2(dot(u, u))));
 }
 
-// |u| beyond which frame_profile is exactly 0 in FP32 (its exponent below -160)
+// |u| beyond which frame_profile is below 2^-127 (its exponent below -127):
+// a subnormal, which the kernels' FP32 arithmetic (subnormals flushed, as
+// the reference's -cl-denorms-are-zero) takes as 0
 static float frame_cutoff(local data* this)
 {
-    float e = (this->lnorm + 160)/1.44269504088896340736f;
+    float e = (this->lnorm + 127)/1.44269504088896340736f;
     return e > 0 ? exp2(0.5f*(log2(e) - this->lscale)/this->hinv) : 0.0f;
 }
 
