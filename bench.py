@@ -468,15 +468,17 @@ def conv_shapes(launches=5):
         for rows in (8, 1):
             ctx.set_tuning(conv_rows=rows)
             it = iter(sets)
-            out["C3"][f"conv_rows_{rows}"] = stage_ms(ctx, lambda: ctx.loglike_batch(next(it)), launches)["convolve"]
+            ms = stage_ms(ctx, lambda: ctx.loglike_batch(next(it)), launches)["convolve"]
+            out["C3"][f"conv_rows_{rows}"] = {"ms": ms, "resolved_conv_rows": ctx.tuning()["conv_rows"]}
     wls, phys = c4_workload()
     with Context([w.spec() for w in wls], max_batch=256, flags=LENSED_TABLE | LENSED_PROFILE) as ctx:
         for rows in (8, 1):
             ctx.set_tuning(conv_rows=rows)
-            out["C4"][f"conv_rows_{rows}"] = stage_ms(ctx, lambda: ctx.loglike_phys(phys), 2)["convolve"]
+            ms = stage_ms(ctx, lambda: ctx.loglike_phys(phys), 2)["convolve"]
+            out["C4"][f"conv_rows_{rows}"] = {"ms": ms, "resolved_conv_rows": ctx.tuning()["conv_rows"]}
     return {"convolve_ms_per_launch": out,
             "note": "conv_rows 8: the 32-row tile; conv_rows 1: the column sweep (the default where every "
-                    "system is culled)"}
+                    "system is culled; C4's 31 x 31 PSFs keep the tile)"}
 
 
 def copy_probe(mib, reps=3):

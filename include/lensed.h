@@ -259,7 +259,7 @@ typedef struct {
                              (8, or 2 when the launch is too small to fill the device); 16 needs 64 + PSF
                              height - 1 rows of LDS and falls back to 8 where that passes 64 KiB; or 1, the
                              column sweep: one wave per 64-column strip of an image runs down the strip,
-                             for launches whose systems all have a rank-1 separable PSF at most 65 wide and
+                             for launches whose systems all have a rank-1 separable PSF of at most 21 x 21 and
                              at most 512 rows (otherwise the default tile) */
     int conv_order;       /* convolution items: 1 candidate-fastest over the grid, 2 each block a run of one
                              candidate's tiles down 64-column strips within its XCD's share (1) */

@@ -13,7 +13,9 @@
 #define LENSED_CONV_RY_TALL 16  // output rows per thread of the 64-row tile (a tuning choice)
 #define LENSED_CONV_SWEEP 1   // conv_rows value of the column sweep: one wave per 64-column strip
 #define LENSED_SWEEP_MAX_H 512  // the sweep's 8-row band flags fit one 64-bit mask
-#define LENSED_SWEEP_MAX_PW 65  // the sweep's staged row: 64 + PW - 1 floats of 128
+#define LENSED_SWEEP_MAX_PSF 21  // the sweep's PSF sides at most: its PH-row window is unrolled, and its
+                                // 2 x 21 taps stay in scalar registers (a 31-row window is no longer
+                                // unrolled by the compiler and indexes registers indirectly)
 // chi^2 partial sums are formed over fixed 8-row x 64-column blocks, so the
 // short tile (4 x 2 rows) is exactly one block, the default one (4 x 8) four
 // and the tall one (4 x 16) eight

@@ -940,7 +940,7 @@ extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) __attribute__((amdgpu_wa
 // 0 without the column pass.  Nothing is reread: each row of the model is
 // loaded once per strip (its halo columns also by the neighbouring strips,
 // which run beside it in the same XCD), against the tile kernel's halo rows.
-#define LENSED_SWEEP_ROW 128  // floats per wave's staged row: 64 + PW - 1 for PW <= 65
+#define LENSED_SWEEP_ROW 96  // floats per wave's staged row: 64 + PW - 1 for PW <= LENSED_SWEEP_MAX_PSF
 
 template <int PW, int PH, bool OUT>
 __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, int strip,
@@ -1120,11 +1120,16 @@ __device__ inline void lensed_conv_sweep_body(const lensed_desc* __restrict__ sy
     const int local = item - d.item_c;
     const int cand = local / d.tiles_x, strip = local - cand * d.tiles_x;
     float* lrow = rows + wave * LENSED_SWEEP_ROW;
+    // (the host launches the sweep only for layouts whose PSFs are all
+    // separable and at most LENSED_SWEEP_MAX_PSF on a side; the others are
+    // not compiled into it)
 #define LENSED_SWEEP_CASE(PW_, PH_, SEP_)                                                                 \
-    if (SEP_ && d.psfw == PW_ && d.psfh == PH_) {                                                       \
-        lensed_conv_sweep_item<PW_, PH_, OUT>(d, cand, strip, model, image, weight, psf, partials,      \
-                                              conv_out, chi2_out, trace, lrow);                         \
-        return;                                                                                         \
+    if constexpr (SEP_ && PW_ <= LENSED_SWEEP_MAX_PSF && PH_ <= LENSED_SWEEP_MAX_PSF) {                 \
+        if (d.psfw == PW_ && d.psfh == PH_) {                                                           \
+            lensed_conv_sweep_item<PW_, PH_, OUT>(d, cand, strip, model, image, weight, psf, partials,  \
+                                                  conv_out, chi2_out, trace, lrow);                     \
+            return;                                                                                     \
+        }                                                                                               \
     }
     LENSED_FOR_PSFS(LENSED_SWEEP_CASE)
 #undef LENSED_SWEEP_CASE

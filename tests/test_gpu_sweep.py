@@ -7,7 +7,7 @@ dumper plane must be the same bits: culled (EPL table, Gaussian source) and
 unculled (SIE) systems, images whose sides are not multiples of 64 or 8,
 odd and even PSF sides, several systems of different sizes and PSFs in one
 launch, and the fallback to the tile where the sweep does not apply (a PSF
-of higher rank).  The FP64 oracle checks the sweep's planes directly on a
+of higher rank, or wider than 21).  The FP64 oracle checks the sweep's planes directly on a
 small case.
 """
 import numpy as np
@@ -102,7 +102,7 @@ def test_sweep_equals_tile_mixed_systems():
     specs, phys = [], []
     rng = np.random.default_rng(29)
     for (h, w), psf, lens in (((64, 64), W.gaussian_psf(15, 2.5), "epl_plus_shear"),
-                              ((128, 200), W.gaussian_psf(31, 3.0), "sie"),
+                              ((128, 200), W.gaussian_psf(20, 3.0), "sie"),
                               ((256, 256), W.gaussian_psf(21, 3.0), "epl_plus_shear"),
                               ((512, 96), W.gaussian_psf(15, 2.5), "epl_plus_shear")):
         wl = c3_like((h, w), psf, seed=int(h + w))
@@ -142,10 +142,11 @@ def test_sweep_against_the_fp64_oracle():
     assert abs(lt - (-0.5 * O.chi2_d(wl.image, wl.weight, model))) <= 1e-2
 
 
-def test_sweep_falls_back_where_it_does_not_apply():
-    """A Moffat PSF (rank above 1) keeps the tile kernel under conv_rows = 1,
-    with unchanged results."""
-    wl = c3_like((96, 96), W.moffat_psf(15, 3.0, 3.0), seed=7)
+@pytest.mark.parametrize("psf", [W.moffat_psf(15, 3.0, 3.0), W.gaussian_psf(31, 3.0)])
+def test_sweep_falls_back_where_it_does_not_apply(psf):
+    """A Moffat PSF (rank above 1) and a Gaussian wider than 21 keep the tile
+    kernel under conv_rows = 1, with unchanged results."""
+    wl = c3_like((96, 96), psf, seed=7)
     cubes = np.random.default_rng(31).random((64, 14))
     with Context(wl.spec(), max_batch=64, flags=LENSED_TABLE) as ctx:
         ctx.set_tuning(conv_rows=8)
