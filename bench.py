@@ -41,8 +41,7 @@ VALU_ISSUE_PEAK = 256 * 2.4e9  # wave instructions per second
 REF_FLOP_PER_RAY = 165
 PROFILES = os.path.join(ROOT, "profiles")
 RENDER_PROFILE = "lensed_render_c3_counters.md"
-CONV_PROFILE = {"lensed_conv_chi2": "lensed_conv_chi2_c3_counters.md",
-                "lensed_conv_chi2_sweep": "lensed_conv_chi2_sweep_c3_counters.md"}
+CONV_PROFILE = "lensed_conv_c3_counters.md"
 
 
 def profile_figure(name, key):
@@ -181,7 +180,7 @@ def render_roofline(kt, tr, K):
     rays = waves * 64 * 49
     s = kt["render_kernel"] / 1e3
     per_sq = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_traced_square")
-    busy = profile_figure(RENDER_PROFILE, "valu_busy_fraction")
+    per_cycle = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_cu_cycle")
     traffic = profile_figure(RENDER_PROFILE, "hbm_bytes_per_launch")
     achieved = per_sq * waves / s / 1e9 if per_sq else None
     peak = VALU_ISSUE_PEAK / 1e9
@@ -193,15 +192,15 @@ def render_roofline(kt, tr, K):
             "avg_launch_ms": round(kt["render_kernel"], 4), "cull_pass_ms": round(kt["cull_pass"], 4),
             "valu_wave_instructions_per_traced_square": per_sq, "squares_traced": waves,
             "rays_traced_per_launch": rays, "rays_traced_frac": round(tr["rays_traced_frac"], 4),
-            "valu_busy_fraction_from_profile": busy,
+            "valu_wave_instructions_per_cu_cycle_from_profile": per_cycle,
             "reference_flop_equivalent": {
                 "tflops": round(REF_FLOP_PER_RAY * rays / s / 1e12, 2),
                 "note": "the reference formulas' 165 flop per traced ray (its tenth-order series); the table "
                         "path computes the same ray in fewer instructions, so this rate is not a fraction of "
                         "any hardware peak"},
-            "note": "frac = VALU wave instructions issued / (256 CUs x 2.4 GHz x kernel time); transcendentals "
-                    "take two issue slots, so the instruction count tops out below 1 (busy fraction from "
-                    "SQ_ACTIVE_INST_VALU in the profile)"}
+            "note": "frac = VALU wave instructions issued / (256 CUs x 2.4 GHz x kernel time), the nominal "
+                    "issue peak; at the clock the kernel runs (GRBM_GUI_ACTIVE in the profile) it issues one "
+                    "VALU instruction in every cycle of every CU"}
 
 
 def conv_roofline(kt, K, conv_rows, npix=65536, taps=21):
@@ -213,7 +212,7 @@ def conv_roofline(kt, K, conv_rows, npix=65536, taps=21):
     s = kt["convolve"] / 1e3
     flop = (2 * 2 * taps + 4) * npix * K  # row pass, column pass, w (m - d)^2
     model_bytes = 4 * npix * K
-    fetch = profile_figure(CONV_PROFILE[kernel], "hbm_bytes_per_launch")
+    fetch = profile_figure(CONV_PROFILE, "hbm_bytes_per_launch_" + ("sweep" if conv_rows == 1 else "tile"))
     hbm = None if fetch is None else fetch * K / 4096
     return {"kernel": kernel, "avg_launch_ms": round(kt["convolve"], 4),
             "algorithmic_tflops": round(flop / s / 1e12, 3),
