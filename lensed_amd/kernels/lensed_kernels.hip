@@ -923,7 +923,8 @@ extern "C" __global__ __launch_bounds__(LENSED_CONV_BS) __attribute__((amdgpu_wa
 }
 
 // ------------------------------------------------------ conv+chi2, sweep
-// Rank-1 separable PSFs, images up to 512 rows: one wave per 64-column strip
+// Rank-1 separable PSFs of at most 21 x 21, images up to 512 rows (the
+// host checks both): one wave per 64-column strip
 // of one candidate's image sweeps the strip from top to bottom.  Per input
 // row, the row pass of the strip's 64 columns, from the row and its PSF halo
 // staged in the wave's LDS row, goes into a window of the last PH row-pass
