@@ -4,6 +4,11 @@
 
 #include <hip/hiprtc.h>
 
+#include <dlfcn.h>
+#include <link.h>
+#include <sys/stat.h>
+
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -57,6 +62,42 @@ bool rtc_compile(const std::string& source, const std::string& name,
     }
     hiprtcDestroyProgram(&prog);
     return ok;
+}
+
+namespace {
+std::string file_identity(const char* path) {
+    std::string s = path;
+    struct stat st;
+    if (stat(path, &st) == 0)
+        s += ":" + std::to_string((long long)st.st_size) + ":" + std::to_string((long long)st.st_mtime);
+    return s;
+}
+
+std::vector<std::string> loaded_comgr() {
+    std::vector<std::string> found;
+    dl_iterate_phdr(
+        [](dl_phdr_info* info, size_t, void* out) {
+            if (info->dlpi_name && strstr(info->dlpi_name, "amd_comgr"))
+                static_cast<std::vector<std::string>*>(out)->push_back(info->dlpi_name);
+            return 0;
+        },
+        &found);
+    std::sort(found.begin(), found.end());
+    return found;
+}
+}  // namespace
+
+std::string compiler_identity() {
+    // hipRTC loads its compiler library at its first build: a tiny one
+    // settles which copy that is before anything is looked up by it
+    if (loaded_comgr().empty())
+        rtc_compile("extern \"C\" __global__ void lensed_probe() {}", "probe.hip", {"-O0"}, nullptr, nullptr);
+    std::string id;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&hiprtcCompileProgram), &info) && info.dli_fname)
+        id += "hiprtc " + file_identity(info.dli_fname);
+    for (const auto& c : loaded_comgr()) id += " comgr " + file_identity(c.c_str());
+    return id;
 }
 
 namespace {
