@@ -177,6 +177,7 @@ struct lensed {
     std::vector<lsd::Layout> noalloc;
     lensed_desc* h_desc = nullptr;
     size_t cap_trace = 0;
+    size_t cap_cull = 0;   // cull work items (LENSED_CULL_GROUP render items each)
     // profiling: +params, -params, set_params, render, convolve, loglike, +loglike_mem, -loglike_mem
     hipEvent_t ev[10] = {};  // the eight stages' bounds, then the end of the cull pass
     double prof_ms[8] = {};

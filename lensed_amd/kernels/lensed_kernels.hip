@@ -254,87 +254,86 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 // evenly (lit squares bunch in the middle of the image, so a fixed
 // item-to-block mapping would leave the blocks of the edge tiles idle).  One
 // flag per square (`trace`) tells the convolution which tiles are all dark.
-// A cull work item is LENSED_CULL_GROUP render items of one candidate; a
-// block takes consecutive ones and stages a candidate's table once for all.
-// Only systems with d.cull (a tabulated EPL lens and one framed source with a
-// cutoff) are tested; the squares of the others are all listed.
+// A cull work item is LENSED_CULL_GROUP render items of one candidate: 256
+// squares, 64 to each wave, which settles its own squares without waiting
+// for the others; a block takes consecutive items and stages a candidate's
+// table once for all.  Only systems with d.cull (a tabulated EPL lens and one
+// framed source with a cutoff) are tested; the squares of the others are all
+// listed.
+static_assert(LENSED_RENDER_BS == 256 && LENSED_CULL_GROUP == 64, "a cull item gives each wave 64 squares");
+
+// The squares of one wave of a cull item: lane i stands for square
+// 64 * wave + i of the item (render item g0 + square / 4, its square
+// square % 4); bit i of the result is set when that square has a lit pixel.
+// x0, y0: the square's first pixel (lane-wise, for the caller's zeroing).
 template <int K>
-__device__ inline void lensed_cull_group(const lensed_desc& d, int cand, int g0, int ng, bool restage,
-                                         const u32* __restrict__ blocks, const float4* __restrict__ tables,
-                                         const float4* __restrict__ npre, float4* tab_lds, int* wave_lit,
-                                         unsigned long long* sq_masks)
+__device__ inline unsigned long long lensed_cull_wave(const lensed_desc& d, int cand, int g0, int ng, int wave,
+                                                      const u32* __restrict__ blocks,
+                                                      const float4* __restrict__ npre, const float4* tab_lds,
+                                                      int& x0, int& y0)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
 #pragma unroll
     for (int i = 0; i < LENSED_DWORDS; ++i) blk[i] = i < d.dwords ? __builtin_amdgcn_readfirstlane(gb[i]) : 0u;
-    if (restage) {  // a new candidate (the previous group's barrier has retired the old table)
-        const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
-        for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
-    }
     const float4* ccg = npre + d.npre_off + (long long)cand * LENSED_NPRE_STRIDE(d.nq) + d.nq;
     lcl::float4 cc[LENSED_NPRE_CULL];
 #pragma unroll
     for (int i = 0; i < LENSED_NPRE_CULL; ++i) cc[i] = lcl::float4(ccg[i].x, ccg[i].y, ccg[i].z, ccg[i].w);
+    const lcl::float4* tab = (const lcl::float4*)tab_lds;
     const float ox = d.pcs[0], oy = d.pcs[1], sx = d.pcs[2], sy = d.pcs[3], qrad = d.qrad;
-    const int ntx = (d.width + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
-    __syncthreads();  // the table is staged; the previous group's wave_lit is read
-    // coarse pass, one lane per 8 x 8 wave square of the group: the bound at
-    // the square's centre with rho grown by the square's half-diagonal holds
-    // for every ray of the square, so a square found dark here needs no
-    // per-pixel test (squares past the image edge are covered too)
-    // A square whose own centre ray lands within the cutoff is kept without
-    // the per-pixel test (most such squares are lit, and keeping a square is
-    // always exact: the render then computes its values, 0 where they are 0)
-    // (computed by the first wave for the block's 64 squares, shared in LDS)
-    const int wave = (int)threadIdx.x >> 6;
-    if (wave == 0) {
-        bool dark_sq = false, lit_sq = false;
-        const int lane = (int)threadIdx.x & 63, t = lane >> 2, w = lane & 3;
-        if (t < ng) {
-            const int tile = g0 + t, by = tile / ntx, bx = tile - by * ntx;
-            const int x0 = bx * LENSED_RENDER_SIDE + (w & 1) * 8, y0 = by * LENSED_RENDER_SIDE + (w >> 1) * 8;
-            const float px = (float)x0 + 3.5f, py = (float)y0 + 3.5f;
-            const float rho = 3.5f * sqrtf(sx * sx + sy * sy) + qrad;
-            const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(ox + sx * px, oy + sy * py), true);
-            dark_sq = lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, cc, rho);
-            lit_sq = !dark_sq && x0 < d.width && y0 < d.height &&
-                     !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, cc, 0.0f);
-        }
-        const unsigned long long dk = __ballot(dark_sq), kp = __ballot(lit_sq);
-        if (lane == 0) {
-            sq_masks[0] = dk;
-            sq_masks[1] = kp;
-        }
+    const int W = d.width, H = d.height;
+    const int ntx = (W + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
+    const int lane = (int)threadIdx.x & 63, sq = wave * 64 + lane, t = sq >> 2, w = sq & 3;
+    // coarse pass, lane = square: the bound at the square's centre with rho
+    // grown by the square's half-diagonal holds for every ray of the square,
+    // so a square found dark here needs no per-pixel test.  A square whose
+    // own centre ray lands within the cutoff is kept without the per-pixel
+    // test (most such squares are lit, and keeping a square is always exact:
+    // the render then computes its values, 0 where they are 0)
+    bool dark_sq = true, lit_sq = false;
+    x0 = y0 = 0;
+    if (t < ng) {
+        const int tile = g0 + t, by = tile / ntx, bx = tile - by * ntx;
+        x0 = bx * LENSED_RENDER_SIDE + (w & 1) * 8;
+        y0 = by * LENSED_RENDER_SIDE + (w >> 1) * 8;
+        const float px = (float)x0 + 3.5f, py = (float)y0 + 3.5f;
+        const float rho = 3.5f * sqrtf(sx * sx + sy * sy) + qrad;
+        const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(ox + sx * px, oy + sy * py), true);
+        // squares past the image edge hold no pixel: dark
+        dark_sq = x0 >= W || y0 >= H || lcl::cull_model<K>(blk, base, tab, cc, rho);
+        lit_sq = !dark_sq && !lcl::cull_model<K>(blk, base, tab, cc, 0.0f);
     }
-    __syncthreads();  // the masks are published (read before the group's last barrier)
-    const unsigned long long dark = sq_masks[0], kept = sq_masks[1];
-    for (int t = 0; t < ng; ++t) {
-        int any = (kept >> (t * 4 + wave)) & 1ull ? 1 : 0;
-        if (!any && !((dark >> (t * 4 + wave)) & 1ull)) {
-            int px, py;
-            bool lit = false;
-            if (lensed_render_pixel(d, g0 + t, wave, px, py)) {
-                const float cx = ox + sx * (float)px, cy = oy + sy * (float)py;
-                const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
-                lit = !lcl::cull_model<K>(blk, base, (const lcl::float4*)tab_lds, cc, qrad);
-            }
-            any = __any(lit) ? 1 : 0;
+    unsigned long long lit = __ballot(lit_sq);
+    unsigned long long todo = __ballot(!dark_sq && !lit_sq);
+    // per-pixel pass over the unsettled squares, lane = pixel
+    while (todo) {
+        const int b = __builtin_ctzll(todo);
+        todo &= todo - 1ull;
+        const int px = __builtin_amdgcn_readlane(x0, b) + (lane & 7);
+        const int py = __builtin_amdgcn_readlane(y0, b) + (lane >> 3);
+        bool l = false;
+        if (px < W && py < H) {
+            const float cx = ox + sx * (float)px, cy = oy + sy * (float)py;
+            const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
+            l = !lcl::cull_model<K>(blk, base, tab, cc, qrad);
         }
-        if ((threadIdx.x & 63) == 0) wave_lit[t * (LENSED_RENDER_BS / 64) + wave] = any;
+        if (__any(l)) lit |= 1ull << b;
     }
-    __syncthreads();  // wave_lit complete; the table may be replaced
+    return lit;
 }
 
 // live[0] = number of listed squares (zeroed by lensed_set_params), then the
 // squares (item * 4 + w, -1 pads an entry), four to an entry, all four of one
 // candidate: the render gives a block one entry and each wave one square, so
 // no wave of the render idles on a dark square next to a lit one.  A block
-// gathers its entries in LDS and reserves room for a full chunk with one
-// atomic, so the counter sees a few thousand atomics per launch rather than
-// one per entry.
+// gathers its entries in LDS and reserves room for its whole entries with
+// one atomic per chunk, so the counter sees a few thousand atomics per
+// launch rather than one per entry.  An entry is padded only where the
+// candidate changes and at the block's end (a chunk's partial last entry
+// waits for the next chunk), so the list holds at most the squares plus
+// three pads per block: the host sizes it so.
 #define LENSED_CULL_CHUNK 2048
-static_assert(LENSED_RENDER_BS == 256, "a render item is four waves");
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const u32* __restrict__ blocks, const float4* __restrict__ tables, const float4* __restrict__ npre,
@@ -343,110 +342,119 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
 {
     extern __shared__ float4 tab_lds[];
     __shared__ int chunk[LENSED_CULL_CHUNK];
-    __shared__ int wave_lit[LENSED_CULL_GROUP * (LENSED_RENDER_BS / 64)];
-    __shared__ int zero_sq[LENSED_CULL_GROUP * (LENSED_RENDER_BS / 64)];
-    __shared__ unsigned long long sq_masks[2];
+    __shared__ int wave_n[2][LENSED_RENDER_BS / 64];
+    __shared__ int at;
     // written[square]: the square's model values may be nonzero (a render
     // wrote it).  Trusted when this layout wrote the model images last and no
     // error plane is asked for (that plane is written only by such launches);
     // otherwise every dark square is zeroed.
     const bool trust = owner[1] != 0 && !errp;
-    __shared__ int at;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = (int)threadIdx.x & 63;
     // the chunk's fill, the same in every thread (what is listed is
     // block-uniform)
-    int n = 0;
+    int n = 0, parity = 0;
     auto flush = [&]() {
+        const int whole = n & ~3;
+        __syncthreads();  // the chunk is complete
+        if (threadIdx.x == 0) at = whole ? atomicAdd(live, whole) : 0;
         __syncthreads();
-        if (threadIdx.x == 0) at = atomicAdd(live, n);
-        __syncthreads();
-        for (int i = threadIdx.x; i < n; i += LENSED_RENDER_BS) live[1 + at + i] = chunk[i];
-        __syncthreads();  // before chunk and at are written again
-        n = 0;
+        for (int i = threadIdx.x; i < whole; i += LENSED_RENDER_BS) live[1 + at + i] = chunk[i];
+        const int rest = n - whole;  // a partial entry moves to the front
+        const int keep = (int)threadIdx.x < rest ? chunk[whole + threadIdx.x] : 0;
+        __syncthreads();  // every read of the chunk and of `at` is done
+        if ((int)threadIdx.x < rest) chunk[threadIdx.x] = keep;
+        n = rest;
     };
-    // a group lists at most 64 squares (16 items x 4), placed at once: lane
-    // i of the first wave stands for square i of the group; every wave forms
-    // the same ballot of the lit ones, so all agree on the count.  A block
-    // takes consecutive groups, so one candidate's groups follow each other:
-    // an entry continues across them and is padded with -1 only where the
-    // candidate changes and before a flush, so the chunk only ever holds
-    // whole entries
-    static_assert(LENSED_CULL_GROUP * 4 == 64 && LENSED_CULL_CHUNK % 4 == 0, "a group's squares in one wave");
-    const int lane = (int)threadIdx.x & 63;
     auto pad = [&]() {
         const int p = (4 - (n & 3)) & 3;
         if ((int)threadIdx.x < p) chunk[n + threadIdx.x] = -1;
         n += p;
     };
-    auto list_group = [&](int square0, bool lit) {
-        if (n + 64 + 3 > LENSED_CULL_CHUNK) {
-            pad();
-            flush();
+    // each wave's lit squares (mask `mine`: squares 64 * wave + i of the
+    // item starting at square0) after those of the waves before it; the
+    // waves' counts meet in LDS (two sets, so one barrier per item suffices:
+    // a set is written again only after every wave has passed the next
+    // item's barrier)
+    auto list_item = [&](int square0, unsigned long long mine) {
+        if (n + LENSED_CULL_GROUP * 4 + 3 > LENSED_CULL_CHUNK) flush();
+        if (lane == 0) wave_n[parity][wave] = __popcll(mine);
+        __syncthreads();
+        int before = 0, total = 0;
+#pragma unroll
+        for (int v = 0; v < LENSED_RENDER_BS / 64; ++v) {
+            const int c = wave_n[parity][v];
+            before += v < wave ? c : 0;
+            total += c;
         }
-        const unsigned long long mask = __ballot(lit);
-        if (threadIdx.x < 64 && lit) chunk[n + __popcll(mask & ((1ull << lane) - 1ull))] = square0 + lane;
-        n += __popcll(mask);
+        if ((mine >> lane) & 1ull)
+            chunk[n + before + __popcll(mine & ((1ull << lane) - 1ull))] = square0 + wave * 64 + lane;
+        n += total;
+        parity ^= 1;
     };
-    int staged = -1;  // first item of the candidate whose squares the chunk ends with
+    int staged = -1;  // first item of the candidate whose table is in LDS and whose squares the chunk ends with
     int first, end;
     lensed_xcd_run(nitems, first, end);
-    for (int w = first; w < end; ++w) {
-        int s = lensed_find_sys<5>(sys, nsys, w);
+    for (int k = first; k < end; ++k) {
+        int s = lensed_find_sys<5>(sys, nsys, k);
         const lensed_desc& d = sys[s];
         const int groups = (d.ntile_r + LENSED_CULL_GROUP - 1) / LENSED_CULL_GROUP;
-        const int local = w - d.item_k;
+        const int local = k - d.item_k;
         const int cand = local / groups;
         const int g0 = (local - cand * groups) * LENSED_CULL_GROUP;
         const int ng = min(LENSED_CULL_GROUP, d.ntile_r - g0);
         const int item0 = d.item_r + cand * d.ntile_r + g0;
         const int key = d.item_r + cand * d.ntile_r;  // block-uniform
-        const bool restage = key != staged;
-        if (restage) pad();  // entries never mix candidates
+        const int sq = wave * 64 + lane;              // this lane's square of the item
+        if (key != staged) {
+            pad();  // entries never mix candidates
+            if (d.cull) {
+                __syncthreads();  // no wave reads the previous candidate's table any more
+                const float4* src = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
+                for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
+                __syncthreads();
+            }
+        }
         staged = key;
         if (!d.cull) {
-            list_group(4 * item0, lane < 4 * ng);
+            list_item(4 * item0, __ballot(sq < 4 * ng));
             continue;
         }
+        unsigned long long lit = 0;
+        int x0 = 0, y0 = 0;
         switch (d.model) {
-#define LENSED_CULL_CASE(K)                                                                 \
-    case K:                                                                                 \
-        lensed_cull_group<K>(d, cand, g0, ng, restage, blocks, tables, npre, tab_lds, wave_lit, sq_masks); \
+#define LENSED_CULL_CASE(K)                                                                         \
+    case K:                                                                                         \
+        lit = lensed_cull_wave<K>(d, cand, g0, ng, wave, blocks, npre, tab_lds, x0, y0);             \
         break;
             LENSED_FOR_MODELS(LENSED_CULL_CASE)
 #undef LENSED_CULL_CASE
             default: break;
         }
-        // the group's per-square flags (for the convolution), one coalesced
-        // store; which dark squares need their zeros written
-        if ((int)threadIdx.x < ng * (LENSED_RENDER_BS / 64)) {
-            const long long sqi = (long long)item0 * (LENSED_RENDER_BS / 64) + threadIdx.x;
-            const int lit = wave_lit[threadIdx.x];
-            trace[sqi] = lit;
-            zero_sq[threadIdx.x] = !lit && (!trust || written[sqi]) ? 1 : 0;
-            written[sqi] = lit;
+        // the squares' flags (for the convolution and the next launch), one
+        // coalesced store per wave; the dark ones that may hold nonzero
+        // values get their zeros, a square at a time, lane = pixel
+        bool z = false;
+        if (sq < 4 * ng) {
+            const long long sqi = (long long)item0 * (LENSED_RENDER_BS / 64) + sq;
+            const int on = (int)((lit >> lane) & 1ull);
+            trace[sqi] = on;
+            z = !on && (!trust || written[sqi]);
+            written[sqi] = on;
         }
-        __syncthreads();  // zero_sq complete
-        // list the lit squares; the dark ones that may hold nonzero values
-        // get their zeros here: a wholly such item in rows of 16 pixels per
-        // 16 threads, otherwise each square by its own wave
-        const int ntx = (d.width + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
-        const int wave = (int)threadIdx.x >> 6;
-        list_group(4 * item0, lane < 4 * ng && wave_lit[lane] != 0);
-        for (int t = 0; t < ng; ++t) {
-            const int* z = zero_sq + t * (LENSED_RENDER_BS / 64);
-            int px = -1, py = -1;
-            if (!(z[0] & z[1] & z[2] & z[3])) {
-                if (z[wave] && !lensed_render_pixel(d, g0 + t, wave, px, py)) px = -1;
-            } else {
-                const int tile = g0 + t, by = tile / ntx, bx = tile - by * ntx;
-                px = bx * LENSED_RENDER_SIDE + ((int)threadIdx.x & 15);
-                py = by * LENSED_RENDER_SIDE + ((int)threadIdx.x >> 4);
-            }
-            if (px >= 0 && px < d.width && py < d.height) {
+        unsigned long long zero = __ballot(z);
+        while (zero) {
+            const int b = __builtin_ctzll(zero);
+            zero &= zero - 1ull;
+            const int px = __builtin_amdgcn_readlane(x0, b) + (lane & 7);
+            const int py = __builtin_amdgcn_readlane(y0, b) + (lane >> 3);
+            if (px < d.width && py < d.height) {
                 const long long o = d.model_off + (long long)cand * d.npix + (long long)py * d.width + px;
                 model[o] = 0.0f;
                 if (errp) errp[o] = 0.0f;
             }
         }
+        list_item(4 * item0, lit);
     }
     pad();
     if (n > 0) flush();
