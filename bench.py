@@ -162,27 +162,27 @@ def kernel_times(wl, K, launches, flags=0):
         for i in range(launches):
             ctx.loglike_batch(sets[1 + i])
         calls, ms = ctx.profile()
-        items, kept, waves, cull1 = ctx.render_stats()
+        items, kept, quads, cull1 = ctx.render_stats()
     n = calls - calls0
     out = {k: (ms[k] - ms0[k]) / n for k in ms}
     out["cull_pass"] = (cull1 - cull0) / n
     out["render_kernel"] = out["render"] - out["cull_pass"]
-    return out, {"render_items": items, "items_kept_by_cull": kept, "waves_traced": waves,
-                 "rays_traced_frac": waves / (items * (256 // 64))}
+    return out, {"render_items": items, "items_kept_by_cull": kept, "quads_traced": quads,
+                 "rays_traced_frac": quads / (items * 16)}
 
 
 def render_roofline(kt, tr, K):
     """The render kernel against its bound, VALU issue.  Its VALU wave
-    instructions per traced 8 x 8 square (49 rays per pixel) come from the
-    rocprofv3 SQ_INSTS_VALU count of the same launch, recorded in profiles/;
-    the squares traced and the kernel's time from this run."""
-    waves = tr["waves_traced"]
-    rays = waves * 64 * 49
+    instructions per traced 4 x 4 quad (16 pixels, 49 rays each) come from
+    the rocprofv3 SQ_INSTS_VALU count of the same launch, recorded in
+    profiles/; the quads traced and the kernel's time from this run."""
+    quads = tr["quads_traced"]
+    rays = quads * 16 * 49
     s = kt["render_kernel"] / 1e3
-    per_sq = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_traced_square")
+    per_q = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_traced_quad")
     per_cycle = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_cu_cycle")
     traffic = profile_figure(RENDER_PROFILE, "hbm_bytes_per_launch")
-    achieved = per_sq * waves / s / 1e9 if per_sq else None
+    achieved = per_q * quads / s / 1e9 if per_q else None
     peak = VALU_ISSUE_PEAK / 1e9
     return {"bound": "valu", "kernel": "lensed_render", "unit": "G wave-instructions/s",
             "achieved": None if achieved is None else round(achieved, 2), "peak": peak,
@@ -190,7 +190,7 @@ def render_roofline(kt, tr, K):
             "traffic": None if traffic is None else int(traffic * K / 4096),
             "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, calibrated)",
             "avg_launch_ms": round(kt["render_kernel"], 4), "cull_pass_ms": round(kt["cull_pass"], 4),
-            "valu_wave_instructions_per_traced_square": per_sq, "squares_traced": waves,
+            "valu_wave_instructions_per_traced_quad": per_q, "quads_traced": quads,
             "rays_traced_per_launch": rays, "rays_traced_frac": round(tr["rays_traced_frac"], 4),
             "valu_wave_instructions_per_cu_cycle_from_profile": per_cycle,
             "reference_flop_equivalent": {
@@ -440,12 +440,12 @@ def c4_launches(launches=3, flags_extra=0):
             for _ in range(launches):
                 ctx.loglike_phys(phys)
             c1, m1 = ctx.profile()
-            items, kept, waves, cull1 = ctx.render_stats()
+            items, kept, quads, cull1 = ctx.render_stats()
         n = c1 - c0
         st = {k: round((m1[k] - m0[k]) / n, 3) for k in m1}
         st["cull_pass"] = round((cull1 - cull0) / n, 3)
         out[name] = {"ms_per_launch": round(sum(st[k] for k in m1), 3), "stages_ms": st,
-                     "render_items": items, "items_kept": kept, "waves_traced": waves}
+                     "render_items": items, "items_kept": kept, "quads_traced": quads}
     npix = sum(wl.image.size for wl in wls)
     return {"workload": f"C4: 64 systems, {npix / 1e6:.2f} Mpx, 256 prior draws each, EPL table path",
             "launches": launches, "paths": out,

@@ -234,11 +234,12 @@ long lensed_profile(const lensed* l, double* ms8);
 
 /* No counterpart in the reference: what the render of the last launch
  * traced.  items = render items (16 x 16 pixel blocks of every candidate),
- * listed = items the pixel cull kept (all of them without culling), waves =
- * 64-pixel waves whose rays were traced; cull_ms = time of the cull pass
- * accumulated under LENSED_PROFILE (part of the render stage of
- * lensed_profile).  Waits for the launch; any output may be NULL. */
-int lensed_render_stats(const lensed* l, long* items, long* listed, long* waves, double* cull_ms);
+ * listed = items the pixel cull kept (all of them without culling), quads =
+ * 4 x 4 pixel quads whose rays were traced (16 per item without culling);
+ * cull_ms = time of the cull pass accumulated under LENSED_PROFILE (part of
+ * the render stage of lensed_profile).  Waits for the launch; any output may
+ * be NULL. */
+int lensed_render_stats(const lensed* l, long* items, long* listed, long* quads, double* cull_ms);
 
 /* ----------------------------------------------------------- launch shapes */
 

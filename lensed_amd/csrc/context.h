@@ -74,7 +74,7 @@ struct Layout {
     bool any_cull = false;          // some system culls pixels before the render (lensed_cull)
     int grid_conv = 0, grid_render = 0;  // resident blocks for this launch's LDS use
     int grid_cull = 0;
-    int list_max = 0;               // the render walks the cull's list below this many lit squares
+    int list_max = 0;               // the render walks the cull's list below this many lit quads
     int conv_rows = 8;              // rows per thread of the convolution tile (2, 8, 16)
     int conv_order = 0;             // convolution item order (kernel argument, 0 or 1)
     lensed_tuning tune{};           // the resolved knobs (what lensed_get_tuning reports)

@@ -6,6 +6,7 @@
 #define LENSED_RENDER_BS 256  // threads (= pixels) per render work item: a 16 x 16 block,
 #define LENSED_RENDER_SIDE 16 // an 8 x 8 square of it per wave
 #define LENSED_CULL_GROUP 64  // render items per pixel-cull work item: 64 squares to each of its 4 waves
+#define LENSED_LIST_ENTRY 16  // 4 x 4 quads per entry of the render list: a render block, four to a wave
 #define LENSED_CONV_TX 64     // conv tile width  (threads along x)
 #define LENSED_CONV_TY 4      // threads along y
 #define LENSED_CONV_RY 8      // output rows per thread, large launches (a multiple of 8)

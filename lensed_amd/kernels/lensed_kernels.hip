@@ -130,11 +130,12 @@ __device__ inline bool lensed_render_pixel(const lensed_desc& d, int tile, int w
     return px < d.width && py < d.height;
 }
 
-// One thread per pixel; the 2-D quadrature loop runs in the thread with the
-// node table in scalar registers and the candidate's object block held in
-// SGPRs (wave-uniform), so the per-ray work is pure VALU.
+// One thread per pixel (px, py; on: the thread has one); the 2-D quadrature
+// loop runs in the thread with the node table in scalar registers and the
+// candidate's object block held in SGPRs (wave-uniform), so the per-ray work
+// is pure VALU.
 template <int K, bool ERR>
-__device__ inline void lensed_render_item(const lensed_desc& d, int cand, int tile, int sq, bool on, bool restage,
+__device__ inline void lensed_render_item(const lensed_desc& d, int cand, int px, int py, bool on, bool restage,
                                           const u32* __restrict__ blocks, const float4* __restrict__ quad,
                                           const float4* __restrict__ tables, const float4* __restrict__ npre,
                                           float4* tab_lds, float* __restrict__ model, float* __restrict__ errp)
@@ -150,8 +151,7 @@ __device__ inline void lensed_render_item(const lensed_desc& d, int cand, int ti
         for (int i = threadIdx.x; i < LENSED_EPL_TABLE_F4; i += LENSED_RENDER_BS) tab_lds[i] = src[i];
         __syncthreads();
     }
-    int px, py;
-    if (on && lensed_render_pixel(d, tile, sq, px, py)) {
+    if (on) {
         const int k = py * d.width + px;
         // pixel centre as kernel/lensed.cl:25, then one ray per node
         const float cx = d.pcs[0] + d.pcs[2] * (float)px;
@@ -195,36 +195,47 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
     float* __restrict__ model, float* __restrict__ errp, int list_max)
 {
     extern __shared__ float4 tab_lds[];
-    // after a cull pass that left fewer than list_max squares lit, only the
-    // lit 8 x 8 squares it listed, four to an entry, all four of one
-    // candidate (live[1..live[0]]: square = item * 4 + w, -1 for none);
-    // otherwise every item, square w to wave w, a culled system's dark
-    // squares (zeroed by the cull) skipped by their flag.  (Walking a list
-    // that holds nearly every square measured much slower than walking the
-    // items: 304 against 221 ms per launch of 64 systems whose sources never
-    // reach exact zero; list_max is three squares in four by default.)
-    const int wave = (int)threadIdx.x >> 6;
+    // after a cull pass that left fewer than list_max quads lit, only the
+    // lit 4 x 4 quads it listed, sixteen to an entry, all of one candidate
+    // (live[1..live[0]]: quad = (item * 4 + w) * 4 + q, -1 for none), four to
+    // each wave, a quad to each 16 lanes; otherwise every item, square w to
+    // wave w, a culled system's dark squares (zeroed by the cull) skipped by
+    // their flag.  (Walking a list that holds nearly every square measured
+    // much slower than walking the items: 304 against 221 ms per launch of 64
+    // systems whose sources never reach exact zero; list_max is three quads in
+    // four by default.)
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     const bool listed = live && live[0] < list_max;
     int first, end;
-    lensed_xcd_run(listed ? live[0] / 4 : nitems, first, end);
+    lensed_xcd_run(listed ? live[0] / LENSED_LIST_ENTRY : nitems, first, end);
     int staged = -1;  // first item of the candidate whose table is in LDS
     for (int j = first; j < end; ++j) {
-        int item = j, mine = j, sq = wave;
-        bool on = true;
+        int item = j, q = -1;
         if (listed) {
-            item = live[1 + 4 * j] >> 2;  // the entry's candidate is that of its first square
-            const int v = __builtin_amdgcn_readfirstlane(live[1 + 4 * j + wave]);
-            on = v >= 0;
-            mine = on ? v >> 2 : item;
-            sq = on ? v & 3 : 0;
+            const int* e = live + 1 + (long long)LENSED_LIST_ENTRY * j;
+            item = e[0] >> 4;                    // the entry's candidate is that of its first quad
+            q = e[4 * wave + (lane >> 4)];       // this lane's quad
         }
         int s = lensed_find_sys<0>(sys, nsys, item);
         const lensed_desc& d = sys[s];
-        if (!listed && live && d.cull)  // flags exist for the culled systems' squares only
-            on = __builtin_amdgcn_readfirstlane(trace[(long long)j * 4 + wave]) != 0;
         int local = item - d.item_r;
         int cand = local / d.ntile_r;
-        int tile = mine - d.item_r - cand * d.ntile_r;
+        int px = 0, py = 0;
+        bool on;
+        if (listed) {
+            // pixel (lane % 16) of quad q % 4 of square (q / 4) % 4 of render item q / 16
+            const int ntx = (d.width + LENSED_RENDER_SIDE - 1) / LENSED_RENDER_SIDE;
+            const int tile = (q >> 4) - d.item_r - cand * d.ntile_r, by = tile / ntx, bx = tile - by * ntx;
+            const int w = (q >> 2) & 3, p = lane & 15;
+            px = bx * LENSED_RENDER_SIDE + (w & 1) * 8 + (q & 1) * 4 + (p & 3);
+            py = by * LENSED_RENDER_SIDE + (w >> 1) * 8 + ((q >> 1) & 1) * 4 + (p >> 2);
+            on = q >= 0 && px < d.width && py < d.height;
+        } else {
+            on = true;
+            if (live && d.cull)  // flags exist for the culled systems' squares only
+                on = __builtin_amdgcn_readfirstlane(trace[(long long)j * 4 + wave]) != 0;
+            on = on && lensed_render_pixel(d, j - d.item_r - cand * d.ntile_r, wave, px, py);
+        }
         const int key = d.item_r + cand * d.ntile_r;  // block-uniform
         const bool restage = key != staged;
         staged = key;
@@ -232,10 +243,10 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 #define LENSED_RENDER_CASE(K)                                                                                \
     case K:                                                                                                  \
         if (errp)                                                                                            \
-            lensed_render_item<K, true>(d, cand, tile, sq, on, restage, blocks, quad, tables, npre, tab_lds, \
+            lensed_render_item<K, true>(d, cand, px, py, on, restage, blocks, quad, tables, npre, tab_lds,   \
                                         model, errp);                                                        \
         else                                                                                                 \
-            lensed_render_item<K, false>(d, cand, tile, sq, on, restage, blocks, quad, tables, npre, tab_lds, \
+            lensed_render_item<K, false>(d, cand, px, py, on, restage, blocks, quad, tables, npre, tab_lds,  \
                                          model, errp);                                                       \
         break;
             LENSED_FOR_MODELS(LENSED_RENDER_CASE)
@@ -247,30 +258,36 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) __attribute__((amdgpu_
 
 // -------------------------------------------------------------------- cull
 // Before the render: for each 8 x 8 square of each render item (a wave's
-// pixels), whether any of its pixels can receive light (lensed_frame_cull in
-// epl_table.h bounds where a pixel's rays land).  Dark squares get their zero
-// model values here (where they may be missing) and are left out of the
-// render; the lit ones go to a list that the render's blocks share out
-// evenly (lit squares bunch in the middle of the image, so a fixed
-// item-to-block mapping would leave the blocks of the edge tiles idle).  One
-// flag per square (`trace`) tells the convolution which tiles are all dark.
+// pixels in the render's walk), and for each 4 x 4 quad of a square, whether
+// any of its pixels can receive light (lensed_frame_cull in epl_table.h
+// bounds where a pixel's rays land).  Dark quads get their zero model values
+// here (where they may be missing) and are left out of the render; the lit
+// ones go to a list that the render's blocks share out evenly, four quads to
+// a wave (lit quads bunch in the middle of the image, so a fixed
+// item-to-block mapping would leave the blocks of the edge tiles idle, and
+// quads rather than squares trace about 8 % fewer rays at C3: the edge of
+// the lit region cuts through squares).  One flag per square (`trace`, its
+// mask of lit quads) tells the convolution which tiles are all dark and the
+// render's walk which squares to trace.
 // A cull work item is LENSED_CULL_GROUP render items of one candidate: 256
 // squares, 64 to each wave, which settles its own squares without waiting
 // for the others; a block takes consecutive items and stages a candidate's
 // table once for all.  Only systems with d.cull (a tabulated EPL lens and one
-// framed source with a cutoff) are tested; the squares of the others are all
+// framed source with a cutoff) are tested; the quads of the others are all
 // listed.
 static_assert(LENSED_RENDER_BS == 256 && LENSED_CULL_GROUP == 64, "a cull item gives each wave 64 squares");
 
+// lanes (pixels y * 8 + x of a square) of its quad q = 2 * (y / 4) + x / 4
+#define LENSED_QUAD_LANES(q) (0x0F0F0F0Full << (((q) & 1) * 4 + ((q) >> 1) * 32))
+
 // The squares of one wave of a cull item: lane i stands for square
 // 64 * wave + i of the item (render item g0 + square / 4, its square
-// square % 4); bit i of the result is set when that square has a lit pixel.
-// x0, y0: the square's first pixel (lane-wise, for the caller's zeroing).
+// square % 4) and returns that square's mask of lit quads.  x0, y0: the
+// square's first pixel (lane-wise, for the caller's zeroing).
 template <int K>
-__device__ inline unsigned long long lensed_cull_wave(const lensed_desc& d, int cand, int g0, int ng, int wave,
-                                                      const u32* __restrict__ blocks,
-                                                      const float4* __restrict__ npre, const float4* tab_lds,
-                                                      int& x0, int& y0)
+__device__ inline int lensed_cull_wave(const lensed_desc& d, int cand, int g0, int ng, int wave,
+                                       const u32* __restrict__ blocks, const float4* __restrict__ npre,
+                                       const float4* tab_lds, int& x0, int& y0)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
@@ -288,9 +305,10 @@ __device__ inline unsigned long long lensed_cull_wave(const lensed_desc& d, int 
     // coarse pass, lane = square: the bound at the square's centre with rho
     // grown by the square's half-diagonal holds for every ray of the square,
     // so a square found dark here needs no per-pixel test.  A square whose
-    // own centre ray lands within the cutoff is kept without the per-pixel
-    // test (most such squares are lit, and keeping a square is always exact:
-    // the render then computes its values, 0 where they are 0)
+    // own centre ray lands within the cutoff is kept whole without the
+    // per-pixel test (most such squares are lit throughout, and keeping a
+    // quad is always exact: the render then computes its values, 0 where
+    // they are 0)
     bool dark_sq = true, lit_sq = false;
     x0 = y0 = 0;
     if (t < ng) {
@@ -304,9 +322,10 @@ __device__ inline unsigned long long lensed_cull_wave(const lensed_desc& d, int 
         dark_sq = x0 >= W || y0 >= H || lcl::cull_model<K>(blk, base, tab, cc, rho);
         lit_sq = !dark_sq && !lcl::cull_model<K>(blk, base, tab, cc, 0.0f);
     }
-    unsigned long long lit = __ballot(lit_sq);
+    int quads = lit_sq ? 15 : 0;
     unsigned long long todo = __ballot(!dark_sq && !lit_sq);
-    // per-pixel pass over the unsettled squares, lane = pixel
+    // per-pixel pass over the unsettled squares, lane = pixel; a quad is lit
+    // where one of its pixels is
     while (todo) {
         const int b = __builtin_ctzll(todo);
         todo &= todo - 1ull;
@@ -318,22 +337,25 @@ __device__ inline unsigned long long lensed_cull_wave(const lensed_desc& d, int 
             const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
             l = !lcl::cull_model<K>(blk, base, tab, cc, qrad);
         }
-        if (__any(l)) lit |= 1ull << b;
+        const unsigned long long m = __ballot(l);
+        const int q = (m & LENSED_QUAD_LANES(0) ? 1 : 0) | (m & LENSED_QUAD_LANES(1) ? 2 : 0) |
+                      (m & LENSED_QUAD_LANES(2) ? 4 : 0) | (m & LENSED_QUAD_LANES(3) ? 8 : 0);
+        if (lane == b) quads = q;
     }
-    return lit;
+    return quads;
 }
 
-// live[0] = number of listed squares (zeroed by lensed_set_params), then the
-// squares (item * 4 + w, -1 pads an entry), four to an entry, all four of one
-// candidate: the render gives a block one entry and each wave one square, so
-// no wave of the render idles on a dark square next to a lit one.  A block
-// gathers its entries in LDS and reserves room for its whole entries with
-// one atomic per chunk, so the counter sees a few thousand atomics per
-// launch rather than one per entry.  An entry is padded only where the
-// candidate changes and at the block's end (a chunk's partial last entry
-// waits for the next chunk), so the list holds at most the squares plus
-// three pads per block: the host sizes it so.
-#define LENSED_CULL_CHUNK 2048
+// live[0] = number of listed quads (zeroed by lensed_set_params), then the
+// quads (square * 4 + q, square = item * 4 + w; -1 pads an entry), sixteen to
+// an entry, all sixteen of one candidate: the render gives a block one entry
+// and each wave four quads, so no wave of the render idles on a dark quad
+// next to a lit one.  A block gathers its entries in LDS and reserves room
+// for its whole entries with one atomic per chunk, so the counter sees a few
+// thousand atomics per launch rather than one per entry.  An entry is padded
+// only where the candidate changes and at the block's end (a chunk's partial
+// last entry waits for the next chunk), so the list holds at most the quads
+// plus fifteen pads per block: the host sizes it so.
+#define LENSED_CULL_CHUNK 4096
 extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const u32* __restrict__ blocks, const float4* __restrict__ tables, const float4* __restrict__ npre,
@@ -344,10 +366,10 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     __shared__ int chunk[LENSED_CULL_CHUNK];
     __shared__ int wave_n[2][LENSED_RENDER_BS / 64];
     __shared__ int at;
-    // written[square]: the square's model values may be nonzero (a render
-    // wrote it).  Trusted when this layout wrote the model images last and no
-    // error plane is asked for (that plane is written only by such launches);
-    // otherwise every dark square is zeroed.
+    // written[square]: the square's quads whose model values may be nonzero
+    // (a render wrote them).  Trusted when this layout wrote the model images
+    // last and no error plane is asked for (that plane is written only by
+    // such launches); otherwise every dark quad is zeroed.
     const bool trust = owner[1] != 0 && !errp;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
@@ -355,7 +377,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     // block-uniform)
     int n = 0, parity = 0;
     auto flush = [&]() {
-        const int whole = n & ~3;
+        const int whole = n & ~(LENSED_LIST_ENTRY - 1);
         __syncthreads();  // the chunk is complete
         if (threadIdx.x == 0) at = whole ? atomicAdd(live, whole) : 0;
         __syncthreads();
@@ -367,18 +389,22 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         n = rest;
     };
     auto pad = [&]() {
-        const int p = (4 - (n & 3)) & 3;
+        const int p = (LENSED_LIST_ENTRY - (n & (LENSED_LIST_ENTRY - 1))) & (LENSED_LIST_ENTRY - 1);
         if ((int)threadIdx.x < p) chunk[n + threadIdx.x] = -1;
         n += p;
     };
-    // each wave's lit squares (mask `mine`: squares 64 * wave + i of the
-    // item starting at square0) after those of the waves before it; the
-    // waves' counts meet in LDS (two sets, so one barrier per item suffices:
-    // a set is written again only after every wave has passed the next
-    // item's barrier)
-    auto list_item = [&](int square0, unsigned long long mine) {
-        if (n + LENSED_CULL_GROUP * 4 + 3 > LENSED_CULL_CHUNK) flush();
-        if (lane == 0) wave_n[parity][wave] = __popcll(mine);
+    // each wave's lit quads (lane i: the quad mask of square 64 * wave + i
+    // of the item starting at square0), square by square, after those of the
+    // waves before it; the waves' counts meet in LDS (two sets, so one
+    // barrier per item suffices: a set is written again only after every
+    // wave has passed the next item's barrier)
+    auto list_item = [&](int square0, int quads) {
+        if (n + LENSED_CULL_GROUP * 16 + LENSED_LIST_ENTRY - 1 > LENSED_CULL_CHUNK) flush();
+        const unsigned long long m0 = __ballot(quads & 1), m1 = __ballot(quads & 2);
+        const unsigned long long m2 = __ballot(quads & 4), m3 = __ballot(quads & 8);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int mine = __popcll(m0) + __popcll(m1) + __popcll(m2) + __popcll(m3);
+        if (lane == 0) wave_n[parity][wave] = mine;
         __syncthreads();
         int before = 0, total = 0;
 #pragma unroll
@@ -387,12 +413,16 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
             before += v < wave ? c : 0;
             total += c;
         }
-        if ((mine >> lane) & 1ull)
-            chunk[n + before + __popcll(mine & ((1ull << lane) - 1ull))] = square0 + wave * 64 + lane;
+        int at_q = n + before + __popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below) +
+                   __popcll(m3 & below);
+        const int qid = (square0 + wave * 64 + lane) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if ((quads >> q) & 1) chunk[at_q++] = qid + q;
         n += total;
         parity ^= 1;
     };
-    int staged = -1;  // first item of the candidate whose table is in LDS and whose squares the chunk ends with
+    int staged = -1;  // first item of the candidate whose table is in LDS and whose quads the chunk ends with
     int first, end;
     lensed_xcd_run(nitems, first, end);
     for (int k = first; k < end; ++k) {
@@ -417,44 +447,45 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         }
         staged = key;
         if (!d.cull) {
-            list_item(4 * item0, __ballot(sq < 4 * ng));
+            list_item(4 * item0, sq < 4 * ng ? 15 : 0);
             continue;
         }
-        unsigned long long lit = 0;
-        int x0 = 0, y0 = 0;
+        int quads = 0, x0 = 0, y0 = 0;
         switch (d.model) {
 #define LENSED_CULL_CASE(K)                                                                         \
     case K:                                                                                         \
-        lit = lensed_cull_wave<K>(d, cand, g0, ng, wave, blocks, npre, tab_lds, x0, y0);             \
+        quads = lensed_cull_wave<K>(d, cand, g0, ng, wave, blocks, npre, tab_lds, x0, y0);           \
         break;
             LENSED_FOR_MODELS(LENSED_CULL_CASE)
 #undef LENSED_CULL_CASE
             default: break;
         }
-        // the squares' flags (for the convolution and the next launch), one
-        // coalesced store per wave; the dark ones that may hold nonzero
-        // values get their zeros, a square at a time, lane = pixel
-        bool z = false;
+        // the squares' flags (for the convolution, the render's walk and the
+        // next launch), one coalesced store per wave; the dark quads that may
+        // hold nonzero values get their zeros, a square at a time, lane =
+        // pixel
+        int zero = 0;
         if (sq < 4 * ng) {
             const long long sqi = (long long)item0 * (LENSED_RENDER_BS / 64) + sq;
-            const int on = (int)((lit >> lane) & 1ull);
-            trace[sqi] = on;
-            z = !on && (!trust || written[sqi]);
-            written[sqi] = on;
+            trace[sqi] = quads;
+            zero = ~quads & (trust ? written[sqi] : 15) & 15;
+            written[sqi] = quads;
         }
-        unsigned long long zero = __ballot(z);
-        while (zero) {
-            const int b = __builtin_ctzll(zero);
-            zero &= zero - 1ull;
+        unsigned long long todo = __ballot(zero != 0);
+        const int my_quad = ((lane >> 5) << 1) | ((lane >> 2) & 1);  // this lane's quad as a pixel of a square
+        while (todo) {
+            const int b = __builtin_ctzll(todo);
+            todo &= todo - 1ull;
             const int px = __builtin_amdgcn_readlane(x0, b) + (lane & 7);
             const int py = __builtin_amdgcn_readlane(y0, b) + (lane >> 3);
-            if (px < d.width && py < d.height) {
+            const int zq = __builtin_amdgcn_readlane(zero, b);
+            if (((zq >> my_quad) & 1) && px < d.width && py < d.height) {
                 const long long o = d.model_off + (long long)cand * d.npix + (long long)py * d.width + px;
                 model[o] = 0.0f;
                 if (errp) errp[o] = 0.0f;
             }
         }
-        list_item(4 * item0, lit);
+        list_item(4 * item0, quads);
     }
     pad();
     if (n > 0) flush();

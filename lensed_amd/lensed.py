@@ -288,13 +288,14 @@ class Context:
         return n.value, ch.value, mx.value
 
     def render_stats(self):
-        """(render items, items the pixel cull kept, waves traced) of the last
-        launch, and the cull pass's accumulated ms under LENSED_PROFILE."""
+        """(render items, items the pixel cull kept, 4 x 4 quads traced) of
+        the last launch, and the cull pass's accumulated ms under
+        LENSED_PROFILE."""
         from ctypes import c_long
-        n, kept, waves, ms = c_long(), c_long(), c_long(), c_double()
-        _check(L.load().lensed_render_stats(self.ptr, byref(n), byref(kept), byref(waves), byref(ms)),
+        n, kept, quads, ms = c_long(), c_long(), c_long(), c_double()
+        _check(L.load().lensed_render_stats(self.ptr, byref(n), byref(kept), byref(quads), byref(ms)),
                "render_stats")
-        return n.value, kept.value, waves.value, ms.value
+        return n.value, kept.value, quads.value, ms.value
 
     def set_tuning(self, **knobs):
         """Launch shapes for the launches from now on (lensed_tuning fields;

@@ -191,6 +191,6 @@ def test_c4_culling_is_exact(c4):
         assert np.all(np.isfinite(out[0][s])), s
         assert np.array_equal(out[0][s], out[1][s]), s
     items = sum(32 * (wl.shape[0] // 16) * (wl.shape[1] // 16) for wl in wls)
-    assert stats[1] == (items, items, 4 * items)
+    assert stats[1] == (items, items, 16 * items)
     sie = sum(32 * (wl.shape[0] // 16) * (wl.shape[1] // 16) for wl in wls if wl.objects[0][1] == "sie")
     assert stats[0][0] == items and sie <= stats[0][1] <= items, (stats, sie)

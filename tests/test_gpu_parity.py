@@ -375,11 +375,11 @@ def test_c3_culling_is_exact(c3):
     print(f"render ms per launch: culled {render_ms[0]:.3f}, every ray {render_ms[1]:.3f}; "
           f"(items, kept, waves traced) {stats}", flush=True)
     assert render_ms[0] < render_ms[1], render_ms
-    # 16 x 16 pixel items of four waves; without culling every wave of
-    # every item is traced, with it a strict subset over prior draws
+    # 16 x 16 pixel items of sixteen 4 x 4 quads; without culling every quad
+    # of every item is traced, with it a strict subset over prior draws
     items = 1024 * 256
-    assert stats[1] == (items, items, 4 * items)
-    assert stats[0][0] == items and 0 < stats[0][1] < items and stats[0][1] <= stats[0][2] <= 4 * stats[0][1]
+    assert stats[1] == (items, items, 16 * items)
+    assert stats[0][0] == items and 0 < stats[0][1] < items and stats[0][1] <= stats[0][2] <= 16 * stats[0][1]
 
 
 @pytest.mark.parametrize("shape,psf_size", [((37, 53), 5), ((70, 33), 5), ((48, 40), 0)])
@@ -416,7 +416,7 @@ def test_culling_ragged_image_is_exact(shape, psf_size):
     for k in pl:
         assert np.array_equal(pl[k], pl0[k]), k
     items = 256 * ((h + 15) // 16) * ((w + 15) // 16)
-    assert stats[1] == (items, items, 4 * items)
+    assert stats[1] == (items, items, 16 * items)
     assert stats[0][0] == items and 0 < stats[0][1] < items, stats
 
 

@@ -224,8 +224,9 @@ def test_walk_over_culled_items_is_exact(c3):
         assert np.all(np.isfinite(a)) and np.array_equal(a, c) and np.array_equal(b, c)
     for k in pn:
         assert np.array_equal(pw[k], pn[k]) and np.array_equal(pl[k], pn[k]), k
-    # the walk traces the same squares as the list; neither traces them all
-    assert sw[2] == sl[2] < sn[2], (sw, sl, sn)
+    # the walk traces every quad of the squares holding a listed quad, the
+    # list only those quads; neither traces them all
+    assert sl[2] <= sw[2] < sn[2] and sw[2] % 4 == 0 and sw[1] == sl[1], (sw, sl, sn)
 
 
 def test_launch_shapes_keep_results_bit_identical(c3):
