@@ -1030,35 +1030,41 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
     for (int i = 0; i < PW; ++i) rfs[i] = rf[i];
 #pragma unroll
     for (int j = 0; j < PH; ++j) cfs[j] = cf[j];
-    auto row_lit = [&](int y) { return y < H && ((lit >> (y >> 3)) & 1ull); };
-    const bool last_lit = row_lit(H - 1);  // the rows below the image repeat the last one
+    const bool last_lit = (lit >> ((H - 1) >> 3)) & 1ull;  // the rows below the image repeat the last one
     // Loads, issued four iterations ahead in iteration order so that the
     // counter waits keep four iterations in flight; every address is a fixed
-    // base plus a wave-uniform row offset (kept by increments) plus the
-    // lane's byte offset.  Model row yf of the staged columns, or row 0 where
-    // yf is dark or past the image (cached; the values are not used), so every
-    // iteration issues the same loads; data and weight of output row yd,
-    // clamped to the image (rows outside it are not used).
+    // base plus a wave-uniform row offset plus the lane's byte offset.  Model
+    // row yf of the staged columns, or row 0 where yf is dark or past the
+    // image (cached; the values are not used), so every iteration issues the
+    // same loads; data and weight of output row yd, clamped to the image
+    // (rows outside it are not used).  Row yf's lit flag (the last row's for
+    // rows below the image) goes into a queue of the four rows in flight, so
+    // that the row pass reads it there instead of testing the band mask again.
     const char* mb = (const char*)m;
     const char* db = (const char*)img;
     const char* wb = (const char*)wgt;
-    unsigned moff = 0, doff = 0;  // byte offsets of the next fetches' rows
-    int yf = 0, yd = -A;          // and the rows themselves
+    const unsigned rowb = 4u * (unsigned)W;
+    unsigned moff = 0;    // byte offset of row yf
+    int yf = 0, yd = -A;  // the next fetches' rows
+    unsigned litq = 0;    // bit i: lit flag of the i-th oldest row in flight
     auto at = [](const char* base, unsigned off) { return *(const float*)(base + off); };
     auto fetch = [&](float& va, float& vb, float& dv, float& wv) {
-        const unsigned mo = row_lit(yf) ? moff : 0u;
+        const bool inside = yf < H, lf = inside && ((lit >> (yf >> 3)) & 1ull);
+        const unsigned mo = lf ? moff : 0u;
+        const unsigned doff = (unsigned)min(max(yd, 0), H - 1) * rowb;
         va = at(mb, mo + ca);
         vb = at(mb, mo + cb);
         dv = at(db, doff + xd);
         wv = at(wb, doff + xd);
-        moff += 4u * (unsigned)W;
+        litq = (litq >> 1) | ((inside ? lf : last_lit) ? 8u : 0u);
+        moff += rowb;
         ++yf;
-        if (yd >= 0 && yd < H - 1) doff += 4u * (unsigned)W;
         ++yd;
     };
     float win[PH];
     float last = 0.0f;  // the row pass of the last image row (the rows below it)
     float s = 0.0f, c = 0.0f;
+    int left = 8;  // output rows still to come in the current 8-row block
     int age = PH;  // input rows since the last lit one (a window is lit while under PH)
     float a0, b0, d0, w0, a1, b1, d1, w1, a2, b2, d2, w2, a3, b3, d3, w3;
     fetch(a0, b0, d0, w0);
@@ -1071,11 +1077,11 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
             const int yi = y0 + k;  // input row (wave-uniform)
             if (yi < H + A) {
                 const float va = a0, vb = b0, dat = d0, wt = w0;
+                const bool lit_in = litq & 1u;  // row yi's flag, the oldest in flight
                 a0 = a1, b0 = b1, d0 = d1, w0 = w1;
                 a1 = a2, b1 = b2, d1 = d2, w1 = w2;
                 a2 = a3, b2 = b3, d2 = d3, w2 = w3;
                 fetch(a3, b3, d3, w3);
-                const bool lit_in = yi < H ? row_lit(yi) : last_lit;
                 const int y = yi - A;  // output row
                 age = lit_in ? 0 : age + 1;
                 float r = last;
@@ -1119,16 +1125,22 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
                         if (chi2_out) chi2_out[kk] = chi;
                     }
                     lensed_two_sum_add(s, c, chi);
-                    if ((y & 7) == 7 || y == H - 1) {
-                        // rows past the last one count as 0, as in the tile kernel's last block
-                        for (int z = (y & 7) + 1; z < 8; ++z) lensed_two_sum_add(s, c, 0.0f);
+                    if (--left == 0) {
                         const double blk = lensed_wave_csum(s, c);
                         if (lane == 0) part[(long long)(y >> 3) * d.tiles_x] = blk;
                         s = c = 0.0f;
+                        left = 8;
                     }
                 }
             }
         }
+    }
+    if (left < 8) {
+        // a last block short of 8 rows: the rows past the last one count as
+        // 0, as in the tile kernel's last block
+        for (int z = 0; z < left; ++z) lensed_two_sum_add(s, c, 0.0f);
+        const double blk = lensed_wave_csum(s, c);
+        if (lane == 0) part[(long long)((H - 1) >> 3) * d.tiles_x] = blk;
     }
 }
 
