@@ -146,21 +146,25 @@ def time_plain(wl, n):
 TUNED = 0  # flags of the timed legs (LENSED_AUTOTUNE, set in main)
 
 
-def kernel_times(wl, K, launches, flags=0):
+def kernel_times(wl, K, launches, flags=0, warm=30):
     """Stage times (ms per launch, hipEvents on the launch stream, fresh
     draws every launch), with the render stage split into the cull pass and
     the render kernel, and what the render traced in the last launch (the
-    workload's seeded candidates)."""
+    workload's seeded candidates).  `warm` launches first: after a pause
+    the first launches run slower (C3's render 2.3 ms falling to 2.0 ms over
+    about ten launches, as the GPU's clock comes up), and the timed loop runs
+    warm."""
     from lensed_amd._lib import LENSED_EMULATE, LENSED_PROFILE
     from lensed_amd.lensed import Context
     emu = EMULATOR if flags & LENSED_EMULATE else None
     with Context(wl.spec(), max_batch=K, flags=LENSED_PROFILE | TUNED | flags, emulator=emu) as ctx:
-        sets = draws(wl, ctx.ndims(), K, launches + 1)
-        ctx.loglike_batch(sets[0])
+        sets = draws(wl, ctx.ndims(), K, warm + launches)
+        for i in range(warm):
+            ctx.loglike_batch(sets[i])
         calls0, ms0 = ctx.profile()
-        cull0 = ctx.render_stats()[3]
+        cull0 = ctx.cull_ms()
         for i in range(launches):
-            ctx.loglike_batch(sets[1 + i])
+            ctx.loglike_batch(sets[warm + i])
         calls, ms = ctx.profile()
         items, kept, quads, cull1 = ctx.render_stats()
     n = calls - calls0
@@ -173,13 +177,18 @@ def kernel_times(wl, K, launches, flags=0):
 
 def render_roofline(kt, tr, K):
     """The render kernel against its bound, VALU issue.  Its VALU wave
-    instructions per traced 4 x 4 quad (16 pixels, 49 rays each) come from
-    the rocprofv3 SQ_INSTS_VALU count of the same launch, recorded in
-    profiles/; the quads traced and the kernel's time from this run."""
+    instructions per traced 4 x 4 quad (16 pixels, 49 rays each), less the
+    transcendental ones (which issue to their own unit beside the VALU: with
+    them the kernel counts 1.07 VALU instructions per CU per cycle, without
+    them 0.95), come from the rocprofv3 SQ_INSTS_VALU and
+    SQ_INSTS_VALU_TRANS_F32 counts of the same launch, recorded in profiles/;
+    the quads traced and the kernel's time from this run."""
     quads = tr["quads_traced"]
     rays = quads * 16 * 49
     s = kt["render_kernel"] / 1e3
-    per_q = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_traced_quad")
+    per_all = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_traced_quad")
+    per_trans = profile_figure(RENDER_PROFILE, "valu_transcendental_wave_instructions_per_traced_quad")
+    per_q = per_all - per_trans if per_all and per_trans is not None else None
     per_cycle = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_cu_cycle")
     traffic = profile_figure(RENDER_PROFILE, "hbm_bytes_per_launch")
     achieved = per_q * quads / s / 1e9 if per_q else None
@@ -190,7 +199,8 @@ def render_roofline(kt, tr, K):
             "traffic": None if traffic is None else int(traffic * K / 4096),
             "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, calibrated)",
             "avg_launch_ms": round(kt["render_kernel"], 4), "cull_pass_ms": round(kt["cull_pass"], 4),
-            "valu_wave_instructions_per_traced_quad": per_q, "quads_traced": quads,
+            "valu_wave_instructions_per_traced_quad": per_all,
+            "valu_transcendental_wave_instructions_per_traced_quad": per_trans, "quads_traced": quads,
             "rays_traced_per_launch": rays, "rays_traced_frac": round(tr["rays_traced_frac"], 4),
             "valu_wave_instructions_per_cu_cycle_from_profile": per_cycle,
             "reference_flop_equivalent": {
@@ -198,9 +208,10 @@ def render_roofline(kt, tr, K):
                 "note": "the reference formulas' 165 flop per traced ray (its tenth-order series); the table "
                         "path computes the same ray in fewer instructions, so this rate is not a fraction of "
                         "any hardware peak"},
-            "note": "frac = VALU wave instructions issued / (256 CUs x 2.4 GHz x kernel time), the nominal "
-                    "issue peak; at the clock the kernel runs (GRBM_GUI_ACTIVE in the profile) it issues one "
-                    "VALU instruction in every cycle of every CU"}
+            "note": "frac = non-transcendental VALU wave instructions issued / (256 CUs x 2.4 GHz x kernel "
+                    "time), the nominal issue peak of the VALU (one wave64 instruction per CU per cycle); at "
+                    "the clock the kernel runs (GRBM_GUI_ACTIVE in the profile) it issues 0.95 of them per CU "
+                    "per cycle, with the transcendentals on their own unit beside them"}
 
 
 def conv_roofline(kt, K, conv_rows, npix=65536, taps=21):
@@ -436,7 +447,7 @@ def c4_launches(launches=3, flags_extra=0):
             ctx.set_tuning(list_fraction=frac)
             ctx.loglike_phys(phys)
             c0, m0 = ctx.profile()
-            cull0 = ctx.render_stats()[3]
+            cull0 = ctx.cull_ms()
             for _ in range(launches):
                 ctx.loglike_phys(phys)
             c1, m1 = ctx.profile()
@@ -496,8 +507,8 @@ def copy_probe(mib, reps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--plain", type=int, default=1000, help="candidates timed on the plain path")
     ap.add_argument("--no-nested", action="store_true", help="skip the end-to-end nested-sampling comparison")

@@ -237,8 +237,8 @@ long lensed_profile(const lensed* l, double* ms8);
  * listed = items the pixel cull kept (all of them without culling), quads =
  * 4 x 4 pixel quads whose rays were traced (16 per item without culling);
  * cull_ms = time of the cull pass accumulated under LENSED_PROFILE (part of
- * the render stage of lensed_profile).  Waits for the launch; any output may
- * be NULL. */
+ * the render stage of lensed_profile).  Any output may be NULL; listed and
+ * quads wait for the launch and read its render list back. */
 int lensed_render_stats(const lensed* l, long* items, long* listed, long* quads, double* cull_ms);
 
 /* ----------------------------------------------------------- launch shapes */

@@ -297,6 +297,13 @@ class Context:
                "render_stats")
         return n.value, kept.value, quads.value, ms.value
 
+    def cull_ms(self):
+        """The cull pass's accumulated ms under LENSED_PROFILE (without
+        reading the render list back)."""
+        ms = c_double()
+        _check(L.load().lensed_render_stats(self.ptr, None, None, None, byref(ms)), "render_stats")
+        return ms.value
+
     def set_tuning(self, **knobs):
         """Launch shapes for the launches from now on (lensed_tuning fields;
         unnamed ones keep their defaults).  Results do not depend on them."""
