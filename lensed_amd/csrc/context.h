@@ -39,6 +39,12 @@ struct System {
     std::vector<ModelObject> objs;
     std::vector<Param> pars;    // object order
     std::vector<size_t> pmap;   // sampler order -> object order
+    // the batched path's prior transform when every prior is uniform or a
+    // delta (xfast): value = xa + u * xs (xs = 0 for a delta), bounds xlo..xhi
+    // (infinite when unbounded), all in sampler order
+    bool xfast = false;
+    std::vector<double> xa, xs, xlo, xhi;
+    std::vector<char> xdelta;
     size_t ndims = 0;
     int model = 0;
     size_t dwords = 0;
@@ -187,7 +193,9 @@ struct lensed {
 
 namespace lsd {
 
-// evaluate ncand[s] parameter vectors (object order, float) per system
+// evaluate ncand[s] parameter vectors (object order, float) per system;
+// phys[s] == nullptr: system s's parameters are already staged as floats in
+// h_params at its descriptor's offset
 int evaluate(::lensed* l, const std::vector<size_t>& ncand, const std::vector<const double*>& phys,
              const std::vector<double*>& lnew, bool emulate = true);
 

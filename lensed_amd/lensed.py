@@ -219,8 +219,8 @@ class Context:
         """ncand unit cubes -> (logL[ncand], physical[ncand, npar]) in one launch per max_batch."""
         cubes = np.ascontiguousarray(cubes, dtype=np.float64)
         n = cubes.shape[0]
-        phys = np.zeros((n, self.npars(s)))
-        lnew = np.zeros(n)
+        phys = np.empty((n, self.npars(s)))  # both filled by the call
+        lnew = np.empty(n)
         _check(L.load().lensed_loglike_batch(self.ptr, s, cubes.ctypes.data_as(POINTER(c_double)), n,
                                              phys.ctypes.data_as(POINTER(c_double)),
                                              lnew.ctypes.data_as(POINTER(c_double))), "loglike_batch")
