@@ -492,39 +492,42 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
 }
 
 // -------------------------------------------------------------- EPL tables
-// One thread per (candidate, row): the tabulated factor W and dW/dp (the
-// tenth-order series times the radial factor, epl_series.h) at both ends of
-// the interval, in FP64 from the candidate's (q, t), turned into cubic
-// Hermite coefficients; the last row is the constant value at p = 1.  With
-// `only` set, just the flagged candidates (the emulator's fallback).
-extern "C" __global__ __launch_bounds__(256) void lensed_epl_tables(
+// One block per candidate: thread i evaluates the tabulated factor W and
+// dW/dp (the tenth-order series times the radial factor, epl_series.h) at
+// node p = i / NT in FP64 from the candidate's (q, t), thread 0 also at
+// p = 1; the nodes meet in LDS, and thread i turns nodes i and i + 1 into
+// the cubic Hermite coefficients of row i (each node evaluated once, not
+// once per interval end); the last row is the constant value at p = 1.
+// Block b is the candidate whose rows start at row item b * (NT + 1) (the
+// systems' rows follow each other).  With `only` set, just the flagged
+// candidates (the emulator's fallback).
+static_assert(LENSED_EPL_NT == 256, "a table block has a thread per interval");
+extern "C" __global__ __launch_bounds__(LENSED_EPL_NT) void lensed_epl_tables(
     const lensed_desc* __restrict__ sys, int nsys, int nitems,
     const float* __restrict__ params, float4* __restrict__ tables, const int* __restrict__ only)
 {
-    int item = blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= nitems) return;
-    int s = lensed_find_sys<4>(sys, nsys, item);
+    __shared__ double nx[LENSED_EPL_ROWS], ny[LENSED_EPL_ROWS], ndx[LENSED_EPL_ROWS], ndy[LENSED_EPL_ROWS];
+    const int row0 = (int)blockIdx.x * LENSED_EPL_ROWS;
+    if (row0 >= nitems) return;
+    int s = lensed_find_sys<4>(sys, nsys, row0);
     const lensed_desc& d = sys[s];
-    int local = item - d.item_t;
-    int cand = local / LENSED_EPL_ROWS;
-    int i = local - cand * LENSED_EPL_ROWS;
-    if (only && !only[d.table_off / LENSED_EPL_TABLE_F4 + cand]) return;
+    const int cand = (row0 - d.item_t) / LENSED_EPL_ROWS;
+    if (only && !only[d.table_off / LENSED_EPL_TABLE_F4 + cand]) return;  // the whole block
     const float* p = params + d.param_off + (long long)cand * d.npar;
-    double q = p[d.epl_q], t = p[d.epl_t];
-    double x0, y0, dx0, dy0, x1, y1, dx1, dy1;
+    const double q = p[d.epl_q], t = p[d.epl_t];
+    const int i = (int)threadIdx.x;
+    lensed_epl_node(q, t, (double)i / LENSED_EPL_NT, nx[i], ny[i], ndx[i], ndy[i]);
+    if (i == 0)
+        lensed_epl_node(q, t, (double)LENSED_EPL_NT / LENSED_EPL_NT, nx[LENSED_EPL_NT], ny[LENSED_EPL_NT],
+                        ndx[LENSED_EPL_NT], ndy[LENSED_EPL_NT]);
+    __syncthreads();
     float4* out = tables + d.table_off + (long long)cand * LENSED_EPL_TABLE_F4;
-    float4& ox = out[LENSED_EPL_AT(0, i)];
-    float4& oy = out[LENSED_EPL_AT(1, i)];
-    if (i == LENSED_EPL_NT) {  // the end point p = 1 as a constant row
-        lensed_epl_node(q, t, 1.0, x1, y1, dx1, dy1);
-        ox = float4((float)x1, 0.0f, 0.0f, 0.0f);
-        oy = float4((float)y1, 0.0f, 0.0f, 0.0f);
-        return;
+    out[LENSED_EPL_AT(0, i)] = lensed_hermite(nx[i], ndx[i], nx[i + 1], ndx[i + 1]);
+    out[LENSED_EPL_AT(1, i)] = lensed_hermite(ny[i], ndy[i], ny[i + 1], ndy[i + 1]);
+    if (i == 0) {  // the end point p = 1 as a constant row
+        out[LENSED_EPL_AT(0, LENSED_EPL_NT)] = float4((float)nx[LENSED_EPL_NT], 0.0f, 0.0f, 0.0f);
+        out[LENSED_EPL_AT(1, LENSED_EPL_NT)] = float4((float)ny[LENSED_EPL_NT], 0.0f, 0.0f, 0.0f);
     }
-    lensed_epl_node(q, t, (double)i / LENSED_EPL_NT, x0, y0, dx0, dy0);
-    lensed_epl_node(q, t, (double)(i + 1) / LENSED_EPL_NT, x1, y1, dx1, dy1);
-    ox = lensed_hermite(x0, dx0, x1, dx1);
-    oy = lensed_hermite(y0, dy0, y1, dy1);
 }
 
 // --------------------------------------------------------------- conv+chi2
