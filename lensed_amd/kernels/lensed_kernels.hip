@@ -287,7 +287,7 @@ static_assert(LENSED_RENDER_BS == 256 && LENSED_CULL_GROUP == 64, "a cull item g
 template <int K>
 __device__ inline int lensed_cull_wave(const lensed_desc& d, int cand, int g0, int ng, int wave,
                                        const u32* __restrict__ blocks, const float4* __restrict__ npre,
-                                       const float4* tab_lds, int& x0, int& y0)
+                                       const float4* tab_lds, int* slots, int& x0, int& y0)
 {
     u32 blk[LENSED_DWORDS];
     const u32* gb = blocks + d.block_off + (long long)cand * d.dwords;
@@ -323,24 +323,64 @@ __device__ inline int lensed_cull_wave(const lensed_desc& d, int cand, int g0, i
         lit_sq = !dark_sq && !lcl::cull_model<K>(blk, base, tab, cc, 0.0f);
     }
     int quads = lit_sq ? 15 : 0;
-    unsigned long long todo = __ballot(!dark_sq && !lit_sq);
-    // per-pixel pass over the unsettled squares, lane = pixel; a quad is lit
-    // where one of its pixels is
-    while (todo) {
-        const int b = __builtin_ctzll(todo);
-        todo &= todo - 1ull;
-        const int px = __builtin_amdgcn_readlane(x0, b) + (lane & 7);
-        const int py = __builtin_amdgcn_readlane(y0, b) + (lane >> 3);
-        bool l = false;
-        if (px < W && py < H) {
-            const float cx = ox + sx * (float)px, cy = oy + sy * (float)py;
-            const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
-            l = !lcl::cull_model<K>(blk, base, tab, cc, qrad);
+    const unsigned long long todo = __ballot(!dark_sq && !lit_sq);
+    const int nu = __popcll(todo);
+    if (nu == 0) return quads;
+    // the unsettled squares (their lanes, in order) through the wave's LDS
+    // slots: one wave writes and reads them, and its LDS accesses complete in
+    // order; the compiler must keep that order
+    asm volatile("" ::: "memory");
+    if ((todo >> lane) & 1ull) slots[__popcll(todo & ((1ull << lane) - 1ull))] = lane;
+    asm volatile("" ::: "memory");
+    // quad pass, lane = (unsettled square, quad), sixteen squares at a time:
+    // the bound at a quad's centre with rho grown by the quad's half-diagonal
+    // settles its dark quads (about half of them at C3); the per-pixel pass,
+    // lane = (open quad, pixel), then takes the open quads four at a time.  A
+    // quad is lit where one of its pixels is, as from the square's pixels.
+    const float rq = 1.5f * sqrtf(sx * sx + sy * sy) + qrad;
+    for (int first = 0; first < nu; first += 16) {
+        const int k = first + (lane >> 2), q = lane & 3;
+        const bool valid = k < nu;
+        const int sl = valid ? slots[k] : 0;  // the square's lane
+        const int qx0 = __builtin_amdgcn_ds_bpermute(sl << 2, x0) + (q & 1) * 4;
+        const int qy0 = __builtin_amdgcn_ds_bpermute(sl << 2, y0) + (q >> 1) * 4;
+        bool open = false;
+        if (valid && qx0 < W && qy0 < H) {
+            const float px = (float)qx0 + 1.5f, py = (float)qy0 + 1.5f;
+            const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(ox + sx * px, oy + sy * py), true);
+            open = !lcl::cull_model<K>(blk, base, tab, cc, rq);
         }
-        const unsigned long long m = __ballot(l);
-        const int q = (m & LENSED_QUAD_LANES(0) ? 1 : 0) | (m & LENSED_QUAD_LANES(1) ? 2 : 0) |
-                      (m & LENSED_QUAD_LANES(2) ? 4 : 0) | (m & LENSED_QUAD_LANES(3) ? 8 : 0);
-        if (lane == b) quads = q;
+        unsigned long long mq = __ballot(open);
+        while (mq) {
+            int b[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                b[j] = mq ? __builtin_ctzll(mq) : -1;
+                mq &= mq - 1ull;  // (0 stays 0)
+            }
+            const int j = lane >> 4, pp = lane & 15;
+            int bx[4], by[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                bx[i] = b[i] < 0 ? 0x3fffffff : __builtin_amdgcn_readlane(qx0, b[i]);
+                by[i] = b[i] < 0 ? 0x3fffffff : __builtin_amdgcn_readlane(qy0, b[i]);
+            }
+            const int px = (j == 0 ? bx[0] : j == 1 ? bx[1] : j == 2 ? bx[2] : bx[3]) + (pp & 3);
+            const int py = (j == 0 ? by[0] : j == 1 ? by[1] : j == 2 ? by[2] : by[3]) + (pp >> 2);
+            bool l = false;
+            if (px < W && py < H) {
+                const float cx = ox + sx * (float)px, cy = oy + sy * (float)py;
+                const lcl::float4 base = lcl::epl_pre_model<K>(blk, lcl::float2(cx, cy), true);
+                l = !lcl::cull_model<K>(blk, base, tab, cc, qrad);
+            }
+            const unsigned long long m = __ballot(l);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (b[i] >= 0 && ((m >> (16 * i)) & 0xFFFFull)) {
+                    const int owner = __builtin_amdgcn_readlane(sl, b[i]);
+                    if (lane == owner) quads |= 1 << (b[i] & 3);
+                }
+        }
     }
     return quads;
 }
@@ -365,6 +405,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     extern __shared__ float4 tab_lds[];
     __shared__ int chunk[LENSED_CULL_CHUNK];
     __shared__ int wave_n[2][LENSED_RENDER_BS / 64];
+    __shared__ int cull_slots[LENSED_RENDER_BS];  // each wave's 64: its unsettled squares
     __shared__ int at;
     // written[square]: the square's quads whose model values may be nonzero
     // (a render wrote them).  Trusted when this layout wrote the model images
@@ -373,6 +414,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
     const bool trust = owner[1] != 0 && !errp;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
+    int* slots = cull_slots + wave * 64;
     // the chunk's fill, the same in every thread (what is listed is
     // block-uniform)
     int n = 0, parity = 0;
@@ -454,7 +496,7 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
         switch (d.model) {
 #define LENSED_CULL_CASE(K)                                                                         \
     case K:                                                                                         \
-        quads = lensed_cull_wave<K>(d, cand, g0, ng, wave, blocks, npre, tab_lds, x0, y0);           \
+        quads = lensed_cull_wave<K>(d, cand, g0, ng, wave, blocks, npre, tab_lds, slots, x0, y0);    \
         break;
             LENSED_FOR_MODELS(LENSED_CULL_CASE)
 #undef LENSED_CULL_CASE
