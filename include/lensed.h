@@ -250,11 +250,11 @@ int lensed_render_stats(const lensed* l, long* items, long* listed, long* quads,
  * setting changes a result: the model planes and every logL are the same
  * bits under all of them. */
 typedef struct {
-    double list_fraction; /* the render walks the cull's list of lit 8 x 8 squares while under this fraction
-                             of all squares is lit, else every item, skipping dark squares by their flag
+    double list_fraction; /* the render walks the cull's list of lit 4 x 4 quads while under this fraction
+                             of all quads is lit, else every item, skipping dark squares by their flag
                              (0.75; negative: never the list, above 1: always) */
     double render_grid;   /* render blocks as a multiple of the resident count (1) */
-    double cull_grid;     /* pixel-cull blocks, the same (1) */
+    double cull_grid;     /* pixel-cull blocks, the same (4) */
     double conv_grid;     /* convolution blocks, the same (1) */
     int conv_rows;        /* output rows per thread of the convolution tile (4 rows of threads): 2, 8 or 16
                              (8, or 2 when the launch is too small to fill the device); 16 needs 64 + PSF

@@ -127,7 +127,7 @@ int autotune(const TuneTimer& timer, bool culls, lensed_tuning* best) {
         if (try_render(c)) return -1;
     }
     if (culls)
-        for (double g : {0.5, 2.0, 4.0}) {
+        for (double g : {1.0, 2.0, 8.0}) {  // (the default is 4)
             lensed_tuning c = cur;
             c.cull_grid = g;
             if (try_render(c)) return -1;
