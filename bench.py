@@ -189,6 +189,7 @@ def render_roofline(kt, tr, K):
     per_all = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_traced_quad")
     per_trans = profile_figure(RENDER_PROFILE, "valu_transcendental_wave_instructions_per_traced_quad")
     per_q = per_all - per_trans if per_all and per_trans is not None else None
+    flop_q = profile_figure(RENDER_PROFILE, "fp32_flop_per_traced_quad")
     per_cycle = profile_figure(RENDER_PROFILE, "valu_wave_instructions_per_cu_cycle")
     traffic = profile_figure(RENDER_PROFILE, "hbm_bytes_per_launch")
     achieved = per_q * quads / s / 1e9 if per_q else None
@@ -203,6 +204,17 @@ def render_roofline(kt, tr, K):
             "valu_transcendental_wave_instructions_per_traced_quad": per_trans, "quads_traced": quads,
             "rays_traced_per_launch": rays, "rays_traced_frac": round(tr["rays_traced_frac"], 4),
             "valu_wave_instructions_per_cu_cycle_from_profile": per_cycle,
+            "fp32_vector": None if not flop_q else {
+                "tflops": round(flop_q * quads / s / 1e12, 2),
+                "frac_of_published_peak": round(flop_q * quads / s / 157.3e12, 4),
+                "frac_of_nonpacked_peak": round(flop_q * quads / s / 78.64e12, 4),
+                "flop_per_ray": round(flop_q / (16 * 49), 1),
+                "note": "FP32 flop the kernel executes, from the rocprofv3 SQ_INSTS_VALU_FMA_F32 (x2), ADD_F32, "
+                        "MUL_F32 and TRANS_F32 counts of the seeded launch (x64 lanes) per traced quad, over "
+                        "this run's quads and kernel time; the published 157.3 TFLOP/s counts packed FP32, "
+                        "which the build disables, so one FP32 operation per lane per cycle (78.6 TFLOP/s) is "
+                        "the ceiling here and the ray's non-FP32 work (LDS, conversions, copysigns) issues "
+                        "between them"},
             "reference_flop_equivalent": {
                 "tflops": round(REF_FLOP_PER_RAY * rays / s / 1e12, 2),
                 "note": "the reference formulas' 165 flop per traced ray (its tenth-order series); the table "

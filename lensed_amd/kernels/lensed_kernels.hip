@@ -1057,6 +1057,7 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
     // lit 8-row bands (bit b: rows 8b .. 8b + 7) over the columns the strip's
     // row pass reads; lane b ORs the flags of band b's squares
     unsigned long long lit = ~0ull;
+    int band = 1;  // lane b: band b's flag (read per row with one readlane)
     if (trace && d.cull) {
         const int c0 = max(x0 - PW / 2, 0), c1 = min(x0 + 63 + (PW - 1 - PW / 2), W - 1);
         int any = 0;
@@ -1068,6 +1069,7 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
             for (int sx = c0 >> 3; sx <= (c1 >> 3); ++sx) any |= f[4 * (sx >> 1) + (sx & 1) + sub];
         }
         lit = __ballot(any != 0);
+        band = any != 0;
     }
     double* part = partials + d.partial_off + (long long)cand * d.npart + strip;
     float rfs[PW], cfs[PH];
@@ -1094,7 +1096,7 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
     unsigned litq = 0;    // bit i: lit flag of the i-th oldest row in flight
     auto at = [](const char* base, unsigned off) { return *(const float*)(base + off); };
     auto fetch = [&](float& va, float& vb, float& dv, float& wv) {
-        const bool inside = yf < H, lf = inside && ((lit >> (yf >> 3)) & 1ull);
+        const bool inside = yf < H, lf = inside && __builtin_amdgcn_readlane(band, min(yf >> 3, 63)) != 0;
         const unsigned mo = lf ? moff : 0u;
         const unsigned doff = (unsigned)min(max(yd, 0), H - 1) * rowb;
         va = at(mb, mo + ca);
