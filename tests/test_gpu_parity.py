@@ -128,6 +128,32 @@ def test_c1_loglike_plain_batch_oracle(c1):
             assert abs(lb[i] - lref) <= 1e-2 + 1e-5 * abs(lref), (i, lb[i], lref)
 
 
+def test_batched_priors_match_the_plain_transform(c1):
+    """The batched path's prior transform gives the plain path's physical
+    parameters and logL bit for bit whatever the priors: all uniform with a
+    delta (one affine loop per candidate, staged straight as floats) and with
+    normal priors (the per-parameter transform); a draw whose normal prior
+    lands outside its parameter's bounds gets logZero on both paths."""
+    wl, pids = c1
+    variants = {
+        "unif+delta": dict(wl.priors, **{"source.n": "2.5"}),
+        "unif+norm": dict(wl.priors, **{"lens.q": "norm 0.8 0.1", "source.n": "norm 2.5 0.3"}),
+    }
+    for name, pri in variants.items():
+        with Context(wl.spec(pri), max_batch=64) as ctx:
+            nd = ctx.ndims()
+            assert nd == (11 if name == "unif+delta" else 12)
+            cubes = np.random.default_rng(7).random((40, nd))
+            cubes[5, :] = 0.999999  # lens.q = 0.8 + 4.75 sigma > 1 under the normal prior
+            lb, pb = ctx.loglike_batch(cubes)
+            for i in range(len(cubes)):
+                lp, pp = ctx.loglike(cubes[i])
+                assert lp == lb[i], (name, i, lp, lb[i])
+                assert np.array_equal(pp, pb[i]), (name, i)
+            assert np.all(np.isfinite(np.delete(lb, 5))), name
+            assert (lb[5] == LOG_ZERO) == (name == "unif+norm"), (name, lb[5])
+
+
 @pytest.fixture(scope="module")
 def c2():
     wl = W.c2()
