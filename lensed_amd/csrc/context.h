@@ -125,7 +125,7 @@ struct lensed {
     hipModule_t module = nullptr;
     hipFunction_t f_set = nullptr, f_render = nullptr, f_conv = nullptr, f_conv_small = nullptr, f_final = nullptr,
                   f_tables = nullptr, f_cull = nullptr, f_conv_tall = nullptr, f_conv_sweep = nullptr,
-                  f_conv_sweep_out = nullptr;
+                  f_conv_sweep_out = nullptr, f_set_tables = nullptr;
     lensed_tuning tune_req{};    // requested launch shapes (zero fields: defaults)
     int cu_count = 1;
     hipStream_t stream = nullptr;

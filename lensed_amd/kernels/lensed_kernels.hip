@@ -78,12 +78,11 @@ __device__ inline void lensed_node_pairs(const lensed_desc& d, u32* blk, const f
     }
 }
 
-extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
-    const lensed_desc* __restrict__ sys, int nsys, int nitems,
-    const float* __restrict__ params, u32* __restrict__ blocks, const float4* __restrict__ quad,
-    float4* __restrict__ npre, int* __restrict__ live, int* __restrict__ owner, int layout)
+__device__ __attribute__((always_inline)) inline void lensed_set_params_item(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems, int item, const float* __restrict__ params,
+    u32* __restrict__ blocks, const float4* __restrict__ quad, float4* __restrict__ npre, int* __restrict__ live,
+    int* __restrict__ owner, int layout)
 {
-    int item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= nitems) return;
     if (item == 0) {
         if (live) live[0] = 0;  // the cull pass's list of lit squares starts empty
@@ -114,6 +113,15 @@ extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
 #pragma unroll
     for (int i = 0; i < LENSED_DWORDS; ++i)
         if (i < d.dwords) out[i] = blk[i];
+}
+
+extern "C" __global__ __launch_bounds__(256) void lensed_set_params(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems,
+    const float* __restrict__ params, u32* __restrict__ blocks, const float4* __restrict__ quad,
+    float4* __restrict__ npre, int* __restrict__ live, int* __restrict__ owner, int layout)
+{
+    lensed_set_params_item(sys, nsys, nitems, (int)(blockIdx.x * blockDim.x + threadIdx.x), params, blocks, quad, npre,
+                           live, owner, layout);
 }
 
 // ------------------------------------------------------------------ render
@@ -544,12 +552,12 @@ extern "C" __global__ __launch_bounds__(LENSED_RENDER_BS) void lensed_cull(
 // systems' rows follow each other).  With `only` set, just the flagged
 // candidates (the emulator's fallback).
 static_assert(LENSED_EPL_NT == 256, "a table block has a thread per interval");
-extern "C" __global__ __launch_bounds__(LENSED_EPL_NT) void lensed_epl_tables(
-    const lensed_desc* __restrict__ sys, int nsys, int nitems,
-    const float* __restrict__ params, float4* __restrict__ tables, const int* __restrict__ only)
+__device__ __attribute__((always_inline)) inline void lensed_epl_tables_block(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems, int b, const float* __restrict__ params,
+    float4* __restrict__ tables, const int* __restrict__ only)
 {
     __shared__ double nx[LENSED_EPL_ROWS], ny[LENSED_EPL_ROWS], ndx[LENSED_EPL_ROWS], ndy[LENSED_EPL_ROWS];
-    const int row0 = (int)blockIdx.x * LENSED_EPL_ROWS;
+    const int row0 = b * LENSED_EPL_ROWS;
     if (row0 >= nitems) return;
     int s = lensed_find_sys<4>(sys, nsys, row0);
     const lensed_desc& d = sys[s];
@@ -570,6 +578,33 @@ extern "C" __global__ __launch_bounds__(LENSED_EPL_NT) void lensed_epl_tables(
         out[LENSED_EPL_AT(0, LENSED_EPL_NT)] = float4((float)nx[LENSED_EPL_NT], 0.0f, 0.0f, 0.0f);
         out[LENSED_EPL_AT(1, LENSED_EPL_NT)] = float4((float)ny[LENSED_EPL_NT], 0.0f, 0.0f, 0.0f);
     }
+}
+
+extern "C" __global__ __launch_bounds__(LENSED_EPL_NT) void lensed_epl_tables(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems,
+    const float* __restrict__ params, float4* __restrict__ tables, const int* __restrict__ only)
+{
+    lensed_epl_tables_block(sys, nsys, nitems, (int)blockIdx.x, params, tables, only);
+}
+
+// lensed_set_params and lensed_epl_tables (every candidate) in one launch:
+// the two read the parameters and write different buffers, so the first
+// set_blocks blocks are set_params' blocks and the rest the tables' blocks,
+// run side by side (set_params' one thread per candidate is latency-bound,
+// the tables' block per candidate fills the device meanwhile).  The same
+// code per thread as the two launches, so the same bits.
+extern "C" __global__ __launch_bounds__(LENSED_EPL_NT) void lensed_set_params_tables(
+    const lensed_desc* __restrict__ sys, int nsys, int nitems,
+    const float* __restrict__ params, u32* __restrict__ blocks, const float4* __restrict__ quad,
+    float4* __restrict__ npre, int* __restrict__ live, int* __restrict__ owner, int layout,
+    int ntable, float4* __restrict__ tables, int set_blocks)
+{
+    const int b = (int)blockIdx.x;
+    if (b < set_blocks)
+        lensed_set_params_item(sys, nsys, nitems, b * LENSED_EPL_NT + (int)threadIdx.x, params, blocks, quad, npre,
+                               live, owner, layout);
+    else
+        lensed_epl_tables_block(sys, nsys, ntable, b - set_blocks, params, tables, nullptr);
 }
 
 // --------------------------------------------------------------- conv+chi2
