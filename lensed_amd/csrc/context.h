@@ -50,6 +50,7 @@ struct System {
     size_t dwords = 0;
     // offsets of this system's constant data on the device
     long long image_off = 0, psf_off = 0;
+    long long dark_off = 0;  // doubles into the context's dark partials (lensed_dark_partials)
     int quad_off = 0;
     int ntile_r = 0;
     int sep_rank = 0;             // separable PSF rank, 0 = direct convolution
@@ -125,7 +126,7 @@ struct lensed {
     hipModule_t module = nullptr;
     hipFunction_t f_set = nullptr, f_render = nullptr, f_conv = nullptr, f_conv_small = nullptr, f_final = nullptr,
                   f_tables = nullptr, f_cull = nullptr, f_conv_tall = nullptr, f_conv_sweep = nullptr,
-                  f_conv_sweep_out = nullptr, f_set_tables = nullptr;
+                  f_conv_sweep_out = nullptr, f_set_tables = nullptr, f_dark = nullptr;
     lensed_tuning tune_req{};    // requested launch shapes (zero fields: defaults)
     int cu_count = 1;
     hipStream_t stream = nullptr;
@@ -174,6 +175,7 @@ struct lensed {
     int* d_owner = nullptr;      // [0] layout that last wrote the model images, [1] whether the
                                  // current launch's layout is that one (set by lensed_set_params)
     int* d_written = nullptr;    // per render square: its model values may be nonzero (lensed_cull)
+    double* d_dark = nullptr;    // per system: the data's chi^2 block partials for a model of 0
     int next_layout_id = 0;
     unsigned long long layout_clock = 0;
     // the entry point that allocates nothing: this device's static scratch,

@@ -57,6 +57,7 @@ struct lensed_desc {
     long long npre_off;                  // float4 into npre[]: node pairs, cull constants ([ncand][nq + 3])
     float qrad;                          // largest distance of a quadrature node from the pixel centre
     int cull;                            // skip pixels whose rays provably all miss the source
+    long long dark_off;                  // doubles into the data's chi^2 block partials for a model of 0 ([npart])
 };
 
 // LDS row stride of the conv tile: 64 + PW - 1 columns plus room for the

@@ -635,6 +635,16 @@ __device__ inline void lensed_two_sum_add(float& s, float& c, float x) {
     c += e;
 }
 
+// The chi^2 term of one pixel, w (m - d)^2 in FP32 (kernel/lensed.cl:50-51),
+// 0 for a lane past the image's right edge; the sweep and the dark partials
+// share it, so that a block with a model of 0 gives the same bits in both.
+__device__ inline float lensed_chi_term(float m, float dat, float wt, bool in)
+{
+#pragma clang fp reassociate(off)
+    const float dm = m - dat;
+    return in ? wt * dm * dm : 0.0f;
+}
+
 // the value of lane ^ o: a swizzle within each 32-lane half for o < 32
 // (bitmask mode: and 0x1f, or 0, xor o), a permute across the halves for 32;
 // no lane arithmetic, unlike a general shuffle
@@ -1068,7 +1078,7 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
                                               const float* __restrict__ weight, const float* __restrict__ psf,
                                               double* __restrict__ partials, float* __restrict__ conv_out,
                                               float* __restrict__ chi2_out, const int* __restrict__ trace,
-                                              float* lrow)
+                                              const double* __restrict__ dark, float* lrow)
 {
 #pragma clang fp reassociate(off)
     constexpr int A = PH - 1 - PH / 2;  // output row y takes row-pass rows y + A down to y + A - (PH - 1)
@@ -1113,6 +1123,32 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
 #pragma unroll
     for (int j = 0; j < PH; ++j) cfs[j] = cf[j];
     const bool last_lit = (lit >> ((H - 1) >> 3)) & 1ull;  // the rows below the image repeat the last one
+    // An output block (8 rows) whose window rows are all dark is the data's
+    // own chi^2 partial, the same for every candidate: dark[] holds it,
+    // computed once from the data (lensed_dark_partials, the same operations
+    // as a dark block here), and the sweep runs only from the first block
+    // with a lit window row to the last.  Block b's windows read input rows
+    // 8b + A - (PH - 1) .. 8b + 7 + A (clamped), in bands b + DLO .. b + DHI.
+    const int nblk = (H + 7) >> 3;
+    const bool skip = !OUT && dark && trace && d.cull;
+    int first = 0, stop = nblk;  // output blocks [first, stop) are swept
+    if (skip) {
+        constexpr int DLO = -((PH / 2 + 7) / 8), DHI = (7 + A) / 8;
+        unsigned long long need = 0;
+#pragma unroll
+        for (int o = DLO; o <= DHI; ++o) need |= o < 0 ? lit << -o : lit >> o;
+        need &= nblk == 64 ? ~0ull : (1ull << nblk) - 1ull;
+        first = need ? __builtin_ctzll(need) : 0;
+        stop = need ? 64 - __builtin_clzll(need) : 0;
+    }
+    // (wave-uniform, and known so: the sweep's row counters stay scalar)
+    first = __builtin_amdgcn_readfirstlane(first);
+    stop = __builtin_amdgcn_readfirstlane(stop);
+    if (skip && lane < nblk && (lane < first || lane >= stop))
+        part[(long long)lane * d.tiles_x] = dark[d.dark_off + (long long)lane * d.tiles_x + strip];
+    const int y_first = 8 * first;                          // first output row swept
+    const int yi0 = max(y_first + A - (PH - 1), 0);         // first input row its window reads
+    const int yi_end = stop > first ? min(8 * stop, H) + A : yi0;  // input rows [yi0, yi_end)
     // Loads, issued four iterations ahead in iteration order so that the
     // counter waits keep four iterations in flight; every address is a fixed
     // base plus a wave-uniform row offset plus the lane's byte offset.  Model
@@ -1126,8 +1162,8 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
     const char* db = (const char*)img;
     const char* wb = (const char*)wgt;
     const unsigned rowb = 4u * (unsigned)W;
-    unsigned moff = 0;    // byte offset of row yf
-    int yf = 0, yd = -A;  // the next fetches' rows
+    unsigned moff = (unsigned)yi0 * rowb;  // byte offset of row yf
+    int yf = yi0, yd = yi0 - A;            // the next fetches' rows
     unsigned litq = 0;    // bit i: lit flag of the i-th oldest row in flight
     auto at = [](const char* base, unsigned off) { return *(const float*)(base + off); };
     auto fetch = [&](float& va, float& vb, float& dv, float& wv) {
@@ -1153,11 +1189,11 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
     fetch(a1, b1, d1, w1);
     fetch(a2, b2, d2, w2);
     fetch(a3, b3, d3, w3);
-    for (int y0 = 0; y0 < H + A; y0 += PH) {
+    for (int y0 = yi0; y0 < yi_end; y0 += PH) {
 #pragma unroll
         for (int k = 0; k < PH; ++k) {
             const int yi = y0 + k;  // input row (wave-uniform)
-            if (yi < H + A) {
+            if (yi < yi_end) {
                 const float va = a0, vb = b0, dat = d0, wt = w0;
                 const bool lit_in = litq & 1u;  // row yi's flag, the oldest in flight
                 a0 = a1, b0 = b1, d0 = d1, w0 = w1;
@@ -1185,13 +1221,13 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
                         for (int i = 1; i < PW; ++i) r = __builtin_fmaf(rfs[i], lrow[lane + PW - 1 - i], r);
                     }
                     last = r;
-                    if (yi == 0) {
+                    if (k == 0 && yi == 0) {  // (only the first unrolled row can be row 0)
 #pragma unroll
                         for (int q = 0; q < PH; ++q) win[q] = r;  // rows above the image are row 0
                     }
                 }
                 win[k] = r;
-                if (y >= 0) {
+                if (y >= y_first) {
                     // the window (input rows yi - PH + 1 .. yi, clamped) holds a
                     // lit row unless the last lit one is PH or more rows back
                     float acc = 0.0f;
@@ -1199,8 +1235,7 @@ __device__ inline void lensed_conv_sweep_item(const lensed_desc& d, int cand, in
 #pragma unroll
                         for (int j = 0; j < PH; ++j) acc = __builtin_fmaf(cfs[j], win[(k - j + PH) % PH], acc);
                     }
-                    const float dm = acc - dat;
-                    const float chi = gx < W ? wt * dm * dm : 0.0f;
+                    const float chi = lensed_chi_term(acc, dat, wt, gx < W);
                     if (OUT && gx < W) {
                         const long long kk = d.model_off + (long long)cand * d.npix + (long long)y * W + gx;
                         if (conv_out) conv_out[kk] = acc;
@@ -1237,7 +1272,8 @@ __device__ inline void lensed_conv_sweep_body(const lensed_desc* __restrict__ sy
                                               const float* __restrict__ model, const float* __restrict__ image,
                                               const float* __restrict__ weight, const float* __restrict__ psf,
                                               double* __restrict__ partials, float* __restrict__ conv_out,
-                                              float* __restrict__ chi2_out, const int* __restrict__ trace)
+                                              float* __restrict__ chi2_out, const int* __restrict__ trace,
+                                              const double* __restrict__ dark)
 {
     __shared__ float rows[4 * LENSED_SWEEP_ROW];
     const int nb = (int)gridDim.x, b = (int)blockIdx.x;
@@ -1261,7 +1297,7 @@ __device__ inline void lensed_conv_sweep_body(const lensed_desc* __restrict__ sy
     if constexpr (SEP_ && PW_ <= LENSED_SWEEP_MAX_PSF && PH_ <= LENSED_SWEEP_MAX_PSF) {                 \
         if (d.psfw == PW_ && d.psfh == PH_) {                                                           \
             lensed_conv_sweep_item<PW_, PH_, OUT>(d, cand, strip, model, image, weight, psf, partials,  \
-                                                  conv_out, chi2_out, trace, lrow);                     \
+                                                  conv_out, chi2_out, trace, dark, lrow);               \
             return;                                                                                     \
         }                                                                                               \
     }
@@ -1274,9 +1310,10 @@ extern "C" __global__ __launch_bounds__(256) void lensed_conv_chi2_sweep(
     const float* __restrict__ model, const float* __restrict__ image,
     const float* __restrict__ weight, const float* __restrict__ psf,
     double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out,
-    const int* __restrict__ trace, int order)
+    const int* __restrict__ trace, int order, const double* __restrict__ dark)
 {
-    lensed_conv_sweep_body<false>(sys, nsys, nitems, model, image, weight, psf, partials, nullptr, nullptr, trace);
+    lensed_conv_sweep_body<false>(sys, nsys, nitems, model, image, weight, psf, partials, nullptr, nullptr, trace,
+                                  dark);
     (void)conv_out, (void)chi2_out, (void)order;
 }
 
@@ -1285,10 +1322,37 @@ extern "C" __global__ __launch_bounds__(256) void lensed_conv_chi2_sweep_out(
     const float* __restrict__ model, const float* __restrict__ image,
     const float* __restrict__ weight, const float* __restrict__ psf,
     double* __restrict__ partials, float* __restrict__ conv_out, float* __restrict__ chi2_out,
-    const int* __restrict__ trace, int order)
+    const int* __restrict__ trace, int order, const double* __restrict__ dark)
 {
-    lensed_conv_sweep_body<true>(sys, nsys, nitems, model, image, weight, psf, partials, conv_out, chi2_out, trace);
-    (void)order;
+    // (every output is stored: no block is taken from the dark partials)
+    lensed_conv_sweep_body<true>(sys, nsys, nitems, model, image, weight, psf, partials, conv_out, chi2_out, trace,
+                                 nullptr);
+    (void)order, (void)dark;
+}
+
+// The chi^2 partial of every 8-row x 64-column block of one system's data
+// for a model of 0, once per data set: the sweep takes it for the blocks a
+// candidate leaves dark.  The operations of a dark block in the sweep: the
+// term w (0 - d)^2 per pixel (lensed_chi_term), TwoSum down each column in
+// row order, rows past the image's last as 0, the compensated wave sum.
+// One wave per block; block b = row block * tiles_x + strip, as in a
+// candidate's partials.
+extern "C" __global__ __launch_bounds__(64) void lensed_dark_partials(
+    const float* __restrict__ image, const float* __restrict__ weight, int W, int H, int tiles_x,
+    double* __restrict__ dark)
+{
+    const int b = (int)blockIdx.x, rb = b / tiles_x, strip = b - rb * tiles_x;
+    const int lane = (int)threadIdx.x & 63, gx = strip * 64 + lane;
+    const int xd = min(gx, W - 1);
+    const int y0 = rb * 8, y1 = min(y0 + 8, H);
+    float s = 0.0f, c = 0.0f;
+    for (int y = y0; y < y1; ++y) {
+        const long long o = (long long)y * W + xd;
+        lensed_two_sum_add(s, c, lensed_chi_term(0.0f, image[o], weight[o], gx < W));
+    }
+    for (int y = y1; y < y0 + 8; ++y) lensed_two_sum_add(s, c, 0.0f);
+    const double v = lensed_wave_csum(s, c);
+    if (lane == 0) dark[b] = v;
 }
 
 // ---------------------------------------------------------------- finalize

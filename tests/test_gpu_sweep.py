@@ -79,6 +79,27 @@ def test_sweep_equals_tile_culled(shape, psf):
         assert np.array_equal(p0[k], p1[k]), k
 
 
+@pytest.mark.parametrize("shape", [(256, 256), (100, 150)])
+def test_sweep_dark_blocks_take_the_data_partials(shape):
+    """Sources from far past the image's edges to its centre: candidates whose
+    lit rows sit at the top, at the bottom, between, or nowhere.  The sweep
+    takes a dark end block's chi^2 partial from the data's own (computed once
+    per context, for a model of 0) and sweeps only the rows between; logL bit
+    for bit with the tile kernel, which computes every block."""
+    wl = c3_like(shape, W.gaussian_psf(21, 3.0), seed=5)
+    h, w = shape
+    wl.priors["source.x"] = f"unif {-0.6 * w} {1.6 * w}"
+    wl.priors["source.y"] = f"unif {-0.6 * h} {1.6 * h}"
+    cubes = np.random.default_rng(29).random((512, 14))
+
+    def run(ctx):
+        return ctx.loglike_batch(cubes)[0]
+
+    l0, l1 = both_ways(wl.spec(), LENSED_TABLE | LENSED_GRAPH, run, 512)
+    assert np.all(np.isfinite(l0))
+    assert np.array_equal(l0, l1), np.max(np.abs(l0 - l1))
+
+
 def test_sweep_equals_tile_unculled_c2():
     """C2 (SIE + shear, bulge + disk + sky, 200 x 200, 21 x 21 PSF): no
     pixel cull, ragged strips and blocks; 1024 prior draws."""
