@@ -207,7 +207,7 @@ def render_roofline(kt, tr, K):
             "fp32_vector": None if not flop_q else {
                 "tflops": round(flop_q * quads / s / 1e12, 2),
                 "frac_of_published_peak": round(flop_q * quads / s / 157.3e12, 4),
-                "frac_of_nonpacked_peak": round(flop_q * quads / s / 78.64e12, 4),
+                "frac_of_nonpacked_peak": round(flop_q * quads / s / (PEAK_FP32_NONPACKED * 1e12), 4),
                 "flop_per_ray": round(flop_q / (16 * 49), 1),
                 "note": "FP32 flop the kernel executes, from the rocprofv3 SQ_INSTS_VALU_FMA_F32 (x2), ADD_F32, "
                         "MUL_F32 and TRANS_F32 counts of the seeded launch (x64 lanes) per traced quad, over "
@@ -227,19 +227,29 @@ def render_roofline(kt, tr, K):
 
 
 def conv_roofline(kt, K, conv_rows, npix=65536, taps=21):
-    """The convolution + chi^2 kernel against both its bounds: the rank-1
-    separable passes' FP32 work and the model bytes it must read.  The
-    kernel is the one the launch shapes chose: the column sweep
-    (conv_rows 1) or the tile."""
-    kernel = "lensed_conv_chi2_sweep" if conv_rows == 1 else "lensed_conv_chi2"
+    """The convolution + chi^2 kernel against its bounds: the VALU issue
+    rate it reaches (its instruction count from the profile), the bytes it
+    reads against HBM, and the dense convolution's FP32 work as an
+    equivalent rate.  The kernel is the one the launch shapes chose: the
+    column sweep (conv_rows 1) or the tile."""
+    sweep = conv_rows == 1
+    kernel = "lensed_conv_chi2_sweep" if sweep else "lensed_conv_chi2"
     s = kt["convolve"] / 1e3
-    flop = (2 * 2 * taps + 4) * npix * K  # row pass, column pass, w (m - d)^2
+    flop = (2 * 2 * taps + 4) * npix * K  # row pass, column pass, w (m - d)^2 of every pixel
     model_bytes = 4 * npix * K
-    fetch = profile_figure(CONV_PROFILE, "hbm_bytes_per_launch_" + ("sweep" if conv_rows == 1 else "tile"))
+    which = "sweep" if sweep else "tile"
+    fetch = profile_figure(CONV_PROFILE, "hbm_bytes_per_launch_" + which)
+    valu = profile_figure(CONV_PROFILE, "valu_wave_instructions_per_launch_" + which)
     hbm = None if fetch is None else fetch * K / 4096
+    valu = None if valu is None else valu * K / 4096
     return {"kernel": kernel, "avg_launch_ms": round(kt["convolve"], 4),
-            "algorithmic_tflops": round(flop / s / 1e12, 3),
-            "frac_fp32_nonpacked": round(flop / s / 1e12 / PEAK_FP32_NONPACKED, 4),
+            "valu_wave_instructions_per_launch": None if valu is None else int(valu),
+            "valu_issue_frac": None if valu is None else round(valu / s / VALU_ISSUE_PEAK, 4),
+            "dense_equivalent_tflops": round(flop / s / 1e12, 3),
+            "dense_equivalent_note": "the separable passes and chi^2 term of every pixel; the sweep skips the "
+                                     "row passes of dark rows, the column passes of dark windows and whole "
+                                     "dark end blocks (whose chi^2 is the data's own), so this is not a "
+                                     "fraction of an FP32 peak",
             "model_bytes_per_launch": model_bytes,
             "hbm_bytes_per_launch": None if hbm is None else int(hbm),
             "hbm_over_model_bytes": None if hbm is None else round(hbm / model_bytes, 3),
